@@ -1,0 +1,33 @@
+"""spla_amd: SPLA's local GEMM engine for AMD MI355X (gfx950).
+
+The product is the C/C++ library ``spla_amd/lib/libspla.so.1`` (SPLA's C ABI
+and C++ API over hand-written HIP kernels); this package is its Python mirror.
+"""
+
+from spla_amd.api import (  # noqa: F401
+    MPI_COMM_SELF,
+    MPI_COMM_WORLD,
+    Context,
+    DistributionType,
+    FillMode,
+    GenericError,
+    GPUAllocationError,
+    GPUError,
+    GPUNoDeviceError,
+    GPUSupportError,
+    InternalError,
+    InvalidAllocatorFunctionError,
+    InvalidHandleError,
+    InvalidParameterError,
+    InvalidPointerError,
+    MatrixDistribution,
+    MPIError,
+    Operation,
+    ProcessingUnit,
+    SplaError,
+    check,
+    gemm,
+    pgemm_sbs,
+    pgemm_ssb,
+    pgemm_ssbtr,
+)

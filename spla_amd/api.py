@@ -1,0 +1,380 @@
+"""Python mirror of SPLA's API over the C ABI of libspla.so.1.
+
+Names, argument order and meaning follow SPLA's C++ API
+(include/spla/{context,matrix_distribution,gemm,pgemm_ssb,pgemm_ssbtr,pgemm_sbs}.hpp);
+errors are raised as the exception class of the returned SplaError code
+(include/spla/exceptions.hpp:41-172). Arrays are numpy arrays (host memory)
+or torch tensors (host or device memory), column-major with explicit leading
+dimensions, exactly as the C pointers they stand for; plain integers are taken
+as raw addresses. The element type selects the s/d/c/z entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+
+import numpy as np
+
+from spla_amd import _lib
+
+
+class ProcessingUnit(enum.IntEnum):
+    HOST = 0
+    GPU = 1
+
+
+class Operation(enum.IntEnum):
+    NONE = 0
+    TRANSPOSE = 1
+    CONJ_TRANSPOSE = 2
+
+
+class FillMode(enum.IntEnum):
+    FULL = 0
+    UPPER = 1
+    LOWER = 2
+
+
+class DistributionType(enum.IntEnum):
+    BLACS_BLOCK_CYCLIC = 0
+    MIRROR = 1
+
+
+class SplaError(enum.IntEnum):
+    SUCCESS = 0
+    UNKNOWN_ERROR = 1
+    INTERNAL_ERROR = 2
+    INVALID_PARAMETER_ERROR = 3
+    INVALID_POINTER_ERROR = 4
+    INVALID_HANDLE_ERROR = 5
+    MPI_ERROR = 6
+    MPI_ALLOCATION_ERROR = 7
+    MPI_THREAD_SUPPORT_ERROR = 8
+    GPU_ERROR = 9
+    GPU_SUPPORT_ERROR = 10
+    GPU_ALLOCATION_ERROR = 11
+    GPU_LAUNCH_ERROR = 12
+    GPU_NO_DEVICE_ERROR = 13
+    GPU_INVALID_VALUE_ERROR = 14
+    GPU_INVALID_DEVICE_POINTER_ERROR = 15
+    GPU_BLAS_ERROR = 16
+    INVALID_ALLOCATOR_FUNCTION = 17
+
+
+class GenericError(Exception):
+    code = SplaError.UNKNOWN_ERROR
+
+
+def _err(name, base, code):
+    return type(name, (base,), {"code": code})
+
+
+InternalError = _err("InternalError", GenericError, SplaError.INTERNAL_ERROR)
+InvalidParameterError = _err("InvalidParameterError", GenericError,
+                             SplaError.INVALID_PARAMETER_ERROR)
+InvalidPointerError = _err("InvalidPointerError", GenericError, SplaError.INVALID_POINTER_ERROR)
+InvalidHandleError = _err("InvalidHandleError", GenericError, SplaError.INVALID_HANDLE_ERROR)
+InvalidAllocatorFunctionError = _err("InvalidAllocatorFunctionError", GenericError,
+                                     SplaError.INVALID_ALLOCATOR_FUNCTION)
+MPIError = _err("MPIError", GenericError, SplaError.MPI_ERROR)
+MPIAllocError = _err("MPIAllocError", MPIError, SplaError.MPI_ALLOCATION_ERROR)
+MPIThreadSupportError = _err("MPIThreadSupportError", MPIError,
+                             SplaError.MPI_THREAD_SUPPORT_ERROR)
+GPUError = _err("GPUError", GenericError, SplaError.GPU_ERROR)
+GPUSupportError = _err("GPUSupportError", GPUError, SplaError.GPU_SUPPORT_ERROR)
+GPUAllocationError = _err("GPUAllocationError", GPUError, SplaError.GPU_ALLOCATION_ERROR)
+GPULaunchError = _err("GPULaunchError", GPUError, SplaError.GPU_LAUNCH_ERROR)
+GPUNoDeviceError = _err("GPUNoDeviceError", GPUError, SplaError.GPU_NO_DEVICE_ERROR)
+GPUInvalidValueError = _err("GPUInvalidValueError", GPUError, SplaError.GPU_INVALID_VALUE_ERROR)
+GPUInvalidDevicePointerError = _err("GPUInvalidDevicePointerError", GPUError,
+                                    SplaError.GPU_INVALID_DEVICE_POINTER_ERROR)
+GPUBlasError = _err("GPUBlasError", GPUError, SplaError.GPU_BLAS_ERROR)
+
+_BY_CODE = {cls.code: cls for cls in (
+    InternalError, InvalidParameterError, InvalidPointerError, InvalidHandleError,
+    InvalidAllocatorFunctionError, MPIError, MPIAllocError, MPIThreadSupportError, GPUError,
+    GPUSupportError, GPUAllocationError, GPULaunchError, GPUNoDeviceError, GPUInvalidValueError,
+    GPUInvalidDevicePointerError, GPUBlasError)}
+
+
+def check(code: int) -> None:
+    """Raises the exception class of a non-zero SplaError code."""
+    if code != 0:
+        cls = _BY_CODE.get(code, GenericError)
+        raise cls(f"SPLA error {SplaError(code).name if code in SplaError._value2member_map_ else code}")
+
+
+MPI_COMM_WORLD = _lib.MPI_COMM_WORLD
+MPI_COMM_SELF = _lib.MPI_COMM_SELF
+
+
+class Context:
+    """spla::Context (include/spla/context.hpp:51-278)."""
+
+    def __init__(self, pu: ProcessingUnit = ProcessingUnit.GPU):
+        self._lib = _lib.lib()
+        self._h = ctypes.c_void_p()
+        check(self._lib.spla_ctx_create(ctypes.byref(self._h), int(pu)))
+        self._keep = []  # user allocator callbacks stay alive with the context
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            check(self._lib.spla_ctx_destroy(ctypes.byref(self._h)))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+    def _get(self, name, ctype=ctypes.c_int):
+        v = ctype()
+        check(getattr(self._lib, f"spla_ctx_{name}")(self._h, ctypes.byref(v)))
+        return v.value
+
+    def processing_unit(self) -> ProcessingUnit:
+        return ProcessingUnit(self._get("processing_unit"))
+
+    def num_tiles(self) -> int:
+        return self._get("num_tiles")
+
+    def tile_size_host(self) -> int:
+        return self._get("tile_size_host")
+
+    def tile_size_gpu(self) -> int:
+        return self._get("tile_size_gpu")
+
+    def op_threshold_gpu(self) -> int:
+        return self._get("op_threshold_gpu")
+
+    def gpu_device_id(self) -> int:
+        return self._get("gpu_device_id")
+
+    def allocated_memory_host(self) -> int:
+        return self._get("allocated_memory_host", ctypes.c_uint64)
+
+    def allocated_memory_pinned(self) -> int:
+        return self._get("allocated_memory_pinned", ctypes.c_uint64)
+
+    def allocated_memory_gpu(self) -> int:
+        return self._get("allocated_memory_gpu", ctypes.c_uint64)
+
+    def set_num_tiles(self, v: int) -> None:
+        check(self._lib.spla_ctx_set_num_tiles(self._h, v))
+
+    def set_tile_size_host(self, v: int) -> None:
+        check(self._lib.spla_ctx_set_tile_size_host(self._h, v))
+
+    def set_tile_size_gpu(self, v: int) -> None:
+        check(self._lib.spla_ctx_set_tile_size_gpu(self._h, v))
+
+    def set_op_threshold_gpu(self, v: int) -> None:
+        check(self._lib.spla_ctx_set_op_threshold_gpu(self._h, v))
+
+    def _set_alloc(self, which, allocate, deallocate):
+        alloc_t = self._lib[f"spla_ctx_set_alloc_{which}"].argtypes[1]
+        free_t = self._lib[f"spla_ctx_set_alloc_{which}"].argtypes[2]
+        a = alloc_t(allocate) if allocate is not None else ctypes.cast(None, alloc_t)
+        f = free_t(deallocate) if deallocate is not None else ctypes.cast(None, free_t)
+        self._keep.append((a, f))
+        check(self._lib[f"spla_ctx_set_alloc_{which}"](self._h, a, f))
+
+    def set_alloc_host(self, allocate, deallocate) -> None:
+        self._set_alloc("host", allocate, deallocate)
+
+    def set_alloc_pinned(self, allocate, deallocate) -> None:
+        self._set_alloc("pinned", allocate, deallocate)
+
+    def set_alloc_gpu(self, allocate, deallocate) -> None:
+        self._set_alloc("gpu", allocate, deallocate)
+
+    # -- extensions (include/spla_amd/spla_amd.h) --
+    def set_stream(self, stream) -> None:
+        """Runs the kernels on a caller stream (torch.cuda.Stream, raw handle or None)."""
+        handle = getattr(stream, "cuda_stream", stream)
+        check(self._lib.spla_amd_ctx_set_stream(self._h, handle or None))
+
+
+class MatrixDistribution:
+    """spla::MatrixDistribution (include/spla/matrix_distribution.hpp:47-222)."""
+
+    def __init__(self, handle):
+        self._lib = _lib.lib()
+        self._h = handle
+
+    @classmethod
+    def create_blacs_block_cyclic(cls, comm: int, order: str, proc_grid_rows: int,
+                                  proc_grid_cols: int, row_block_size: int,
+                                  col_block_size: int) -> "MatrixDistribution":
+        h = ctypes.c_void_p()
+        check(_lib.lib().spla_mat_dis_create_block_cyclic(
+            ctypes.byref(h), comm, order.encode()[:1], proc_grid_rows, proc_grid_cols,
+            row_block_size, col_block_size))
+        return cls(h)
+
+    @classmethod
+    def create_blacs_block_cyclic_from_mapping(cls, comm: int, mapping, proc_grid_rows: int,
+                                               proc_grid_cols: int, row_block_size: int,
+                                               col_block_size: int) -> "MatrixDistribution":
+        h = ctypes.c_void_p()
+        arr = None
+        if mapping is not None:
+            arr = (ctypes.c_int * len(mapping))(*mapping)
+        check(_lib.lib().spla_mat_dis_create_blacs_block_cyclic_from_mapping(
+            ctypes.byref(h), comm, arr, proc_grid_rows, proc_grid_cols, row_block_size,
+            col_block_size))
+        return cls(h)
+
+    @classmethod
+    def create_mirror(cls, comm: int) -> "MatrixDistribution":
+        h = ctypes.c_void_p()
+        check(_lib.lib().spla_mat_dis_create_mirror(ctypes.byref(h), comm))
+        return cls(h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            check(self._lib.spla_mat_dis_destroy(ctypes.byref(self._h)))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+    def _get(self, name, ctype=ctypes.c_int):
+        v = ctype()
+        check(getattr(self._lib, f"spla_mat_dis_{name}")(self._h, ctypes.byref(v)))
+        return v.value
+
+    def proc_grid_rows(self) -> int:
+        return self._get("proc_grid_rows")
+
+    def proc_grid_cols(self) -> int:
+        return self._get("proc_grid_cols")
+
+    def row_block_size(self) -> int:
+        return self._get("row_block_size")
+
+    def col_block_size(self) -> int:
+        return self._get("col_block_size")
+
+    def type(self) -> DistributionType:
+        return DistributionType(self._get("type"))
+
+    def comm(self) -> int:
+        return self._get("comm")
+
+    def set_row_block_size(self, v: int) -> None:
+        check(self._lib.spla_mat_dis_set_row_block_size(self._h, v))
+
+    def set_col_block_size(self, v: int) -> None:
+        check(self._lib.spla_mat_dis_set_col_block_size(self._h, v))
+
+
+# ---------------------------------------------------------------------------
+# array plumbing
+
+_PREFIX = {np.dtype(np.float32): "s", np.dtype(np.float64): "d",
+           np.dtype(np.complex64): "c", np.dtype(np.complex128): "z"}
+
+
+def _dtype_of(x):
+    if isinstance(x, np.ndarray):
+        return x.dtype
+    try:
+        import torch
+        if isinstance(x, torch.Tensor):
+            return {torch.float32: np.dtype(np.float32), torch.float64: np.dtype(np.float64),
+                    torch.complex64: np.dtype(np.complex64),
+                    torch.complex128: np.dtype(np.complex128)}[x.dtype]
+    except ImportError:  # pragma: no cover
+        pass
+    return None
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data if x.size else None
+    data_ptr = getattr(x, "data_ptr", None)
+    if data_ptr is not None:
+        return data_ptr() or None
+    raise TypeError(f"unsupported array type {type(x)}")
+
+
+def _prefix(dtype, *arrays):
+    if dtype is None:
+        for a in arrays:
+            dt = _dtype_of(a)
+            if dt is not None:
+                dtype = dt
+                break
+    dtype = np.dtype(dtype if dtype is not None else np.float64)
+    if dtype not in _PREFIX:
+        raise TypeError(f"unsupported element type {dtype}")
+    return _PREFIX[dtype], dtype
+
+
+def _scalar(prefix, v):
+    if prefix == "s":
+        return ctypes.c_float(float(np.real(v)))
+    if prefix == "d":
+        return ctypes.c_double(float(np.real(v)))
+    ct = np.complex64 if prefix == "c" else np.complex128
+    buf = np.array([v], dtype=ct)
+    return buf  # kept alive by the caller frame
+
+
+def _scalar_arg(prefix, buf):
+    return buf if prefix in "sd" else ctypes.c_void_p(buf.ctypes.data)
+
+
+def gemm(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx: Context, dtype=None):
+    """spla::gemm (include/spla/gemm.hpp:66-100): C <- alpha op(A) op(B) + beta C."""
+    p, _ = _prefix(dtype, C, A, B)
+    a, b = _scalar(p, alpha), _scalar(p, beta)
+    check(getattr(ctx._lib, f"spla_{p}gemm")(
+        int(opA), int(opB), m, n, k, _scalar_arg(p, a), _ptr(A), lda, _ptr(B), ldb,
+        _scalar_arg(p, b), _ptr(C), ldc, ctx.handle))
+
+
+def pgemm_ssb(m, n, kLocal, opA, alpha, A, lda, B, ldb, beta, C, ldc, cRowOffset, cColOffset,
+              distC: MatrixDistribution, ctx: Context, dtype=None):
+    """spla::pgemm_ssb (include/spla/pgemm_ssb.hpp:88-120)."""
+    p, _ = _prefix(dtype, C, A, B)
+    a, b = _scalar(p, alpha), _scalar(p, beta)
+    check(getattr(ctx._lib, f"spla_p{p}gemm_ssb")(
+        m, n, kLocal, int(opA), _scalar_arg(p, a), _ptr(A), lda, _ptr(B), ldb, _scalar_arg(p, b),
+        _ptr(C), ldc, cRowOffset, cColOffset, distC.handle, ctx.handle))
+
+
+def pgemm_ssbtr(m, n, kLocal, opA, alpha, A, lda, B, ldb, beta, C, ldc, cRowOffset, cColOffset,
+                cFillMode, distC: MatrixDistribution, ctx: Context, dtype=None):
+    """spla::pgemm_ssbtr (include/spla/pgemm_ssbtr.hpp:94-128)."""
+    p, _ = _prefix(dtype, C, A, B)
+    a, b = _scalar(p, alpha), _scalar(p, beta)
+    check(getattr(ctx._lib, f"spla_p{p}gemm_ssbtr")(
+        m, n, kLocal, int(opA), _scalar_arg(p, a), _ptr(A), lda, _ptr(B), ldb, _scalar_arg(p, b),
+        _ptr(C), ldc, cRowOffset, cColOffset, int(cFillMode), distC.handle, ctx.handle))
+
+
+def pgemm_sbs(mLocal, n, k, alpha, A, lda, B, ldb, bRowOffset, bColOffset,
+              distB: MatrixDistribution, beta, C, ldc, ctx: Context, dtype=None):
+    """spla::pgemm_sbs (include/spla/pgemm_sbs.hpp:86-118)."""
+    p, _ = _prefix(dtype, C, A, B)
+    a, b = _scalar(p, alpha), _scalar(p, beta)
+    check(getattr(ctx._lib, f"spla_p{p}gemm_sbs")(
+        mLocal, n, k, _scalar_arg(p, a), _ptr(A), lda, _ptr(B), ldb, bRowOffset, bColOffset,
+        distB.handle, _scalar_arg(p, b), _ptr(C), ldc, ctx.handle))

@@ -1,0 +1,85 @@
+// Internal state of spla::Context: tunables with SPLA's defaults and checks
+// (src/spla/context_internal.hpp:58-140), memory resources, HIP streams and
+// the tile-plan cache. Streams are created lazily on first GPU use and reused
+// across calls. A context is not thread safe (as in SPLA); the memory
+// resources are.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+
+#include "runtime/allocators.hpp"
+#include "runtime/tuner.hpp"
+#include "spla/types.h"
+
+namespace spla {
+
+class ContextInternal {
+public:
+  explicit ContextInternal(SplaProcessingUnit pu);
+  ~ContextInternal();
+
+  ContextInternal(const ContextInternal&) = delete;
+  ContextInternal& operator=(const ContextInternal&) = delete;
+
+  SplaProcessingUnit processing_unit() const { return pu_; }
+  int num_tiles() const { return numTiles_; }
+  int tile_size_host() const { return tileSizeHost_; }
+  int tile_size_gpu() const { return tileSizeGPU_; }
+  int op_threshold_gpu() const { return opThresholdGPU_; }
+  int gpu_device_id() const { return deviceId_; }
+
+  void set_num_tiles(int v);
+  void set_tile_size_host(int v);
+  void set_tile_size_gpu(int v);
+  void set_op_threshold_gpu(int v);
+
+  const std::shared_ptr<spla_amd::MemoryResource>& host_memory() const { return host_; }
+  const std::shared_ptr<spla_amd::MemoryResource>& pinned_memory() const { return pinned_; }
+  const std::shared_ptr<spla_amd::MemoryResource>& gpu_memory() const { return gpu_; }
+  void set_host_memory(std::shared_ptr<spla_amd::MemoryResource> r) { host_ = std::move(r); }
+  void set_pinned_memory(std::shared_ptr<spla_amd::MemoryResource> r) { pinned_ = std::move(r); }
+  void set_gpu_memory(std::shared_ptr<spla_amd::MemoryResource> r) { gpu_ = std::move(r); }
+
+  // Device the engine runs on: the device current at creation for a GPU
+  // context, the device current at first use for a host context.
+  int device();
+
+  // Device resources; created on first use (the caller makes device() current).
+  hipStream_t compute_stream();
+  // Caller-provided stream for the kernels (nullptr: the context's own).
+  void set_user_stream(hipStream_t s) { userStream_ = s; }
+  hipStream_t copy_stream();
+  int num_cus();
+
+  spla_amd::Tuner& tuner() { return tuner_; }
+
+private:
+  void ensure_device_resources();
+
+  SplaProcessingUnit pu_;
+  int numTiles_ = 4;
+  int tileSizeHost_;
+  int tileSizeGPU_ = 2048;
+  int opThresholdGPU_ = 2000000;
+  int deviceId_ = 0;
+  bool deviceKnown_ = false;
+
+  std::shared_ptr<spla_amd::MemoryResource> host_;
+  std::shared_ptr<spla_amd::MemoryResource> pinned_;
+  std::shared_ptr<spla_amd::MemoryResource> gpu_;
+
+  hipStream_t compute_ = nullptr;
+  hipStream_t userStream_ = nullptr;
+  hipStream_t copy_ = nullptr;
+  int numCUs_ = 0;
+
+  spla_amd::Tuner tuner_;
+};
+
+}  // namespace spla
