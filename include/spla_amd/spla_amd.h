@@ -22,6 +22,13 @@ extern "C" {
  */
 SPLA_EXPORT SplaError spla_amd_ctx_set_stream(SplaContext ctx, void* hipStream);
 
+/*
+ * Target size in bytes of one k-chunk of host-resident operands streamed to
+ * the GPU (default 256 MiB; at least 64 reduction indices per chunk).
+ * Smaller chunks start the pipeline earlier, larger ones run longer kernels.
+ */
+SPLA_EXPORT SplaError spla_amd_ctx_set_stream_chunk_bytes(SplaContext ctx, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,10 @@ def cpu_path() -> dict:
         return _cpu_path_info
     h = lib()
     cands = []
+    # MKL's default Intel-OpenMP layer conflicts with the GNU OpenMP runtime that
+    # torch loads into the same process (wrong complex results, serial speed);
+    # the GNU layer shares that runtime.
+    os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
     mkl = "/opt/conda/lib/libmkl_rt.so"
     if os.path.exists(mkl):
         cands.append((mkl, b"", b"MKL_Set_Num_Threads", b"mkl_get_max_threads", "MKL"))

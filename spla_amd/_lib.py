@@ -81,6 +81,7 @@ def _declare(lib: ctypes.CDLL) -> None:
 
     # include/spla_amd/spla_amd.h
     fn("spla_amd_ctx_set_stream", c_void_p, c_void_p)
+    fn("spla_amd_ctx_set_stream_chunk_bytes", c_void_p, ctypes.c_size_t)
 
 
 def lib() -> ctypes.CDLL:

@@ -177,12 +177,12 @@ class Context:
         check(self._lib.spla_ctx_set_op_threshold_gpu(self._h, v))
 
     def _set_alloc(self, which, allocate, deallocate):
-        alloc_t = self._lib[f"spla_ctx_set_alloc_{which}"].argtypes[1]
-        free_t = self._lib[f"spla_ctx_set_alloc_{which}"].argtypes[2]
+        fn = getattr(self._lib, f"spla_ctx_set_alloc_{which}")
+        alloc_t, free_t = fn.argtypes[1], fn.argtypes[2]
         a = alloc_t(allocate) if allocate is not None else ctypes.cast(None, alloc_t)
         f = free_t(deallocate) if deallocate is not None else ctypes.cast(None, free_t)
         self._keep.append((a, f))
-        check(self._lib[f"spla_ctx_set_alloc_{which}"](self._h, a, f))
+        check(fn(self._h, a, f))
 
     def set_alloc_host(self, allocate, deallocate) -> None:
         self._set_alloc("host", allocate, deallocate)
@@ -198,6 +198,10 @@ class Context:
         """Runs the kernels on a caller stream (torch.cuda.Stream, raw handle or None)."""
         handle = getattr(stream, "cuda_stream", stream)
         check(self._lib.spla_amd_ctx_set_stream(self._h, handle or None))
+
+    def set_stream_chunk_bytes(self, nbytes: int) -> None:
+        """Target bytes of one streamed k-chunk of host-resident operands."""
+        check(self._lib.spla_amd_ctx_set_stream_chunk_bytes(self._h, int(nbytes)))
 
 
 class MatrixDistribution:

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -54,6 +55,10 @@ public:
   hipStream_t compute_stream();
   // Caller-provided stream for the kernels (nullptr: the context's own).
   void set_user_stream(hipStream_t s) { userStream_ = s; }
+
+  // Target bytes of one streamed k-chunk of host-resident operands.
+  std::size_t stream_chunk_bytes() const { return streamChunkBytes_; }
+  void set_stream_chunk_bytes(std::size_t b) { streamChunkBytes_ = b; }
   hipStream_t copy_stream();
   int num_cus();
 
@@ -76,6 +81,7 @@ private:
 
   hipStream_t compute_ = nullptr;
   hipStream_t userStream_ = nullptr;
+  std::size_t streamChunkBytes_ = std::size_t(256) << 20;
   hipStream_t copy_ = nullptr;
   int numCUs_ = 0;
 

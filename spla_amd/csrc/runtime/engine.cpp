@@ -270,8 +270,8 @@ void gemm_gpu_f64(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, in
     const bool kMajorA = opA != SPLA_OP_NONE;
     const bool kMajorB = opB == SPLA_OP_NONE;
     const std::size_t bytesPerK = ((hostA ? m : 0) + (hostB ? n : 0)) * sizeof(T);
-    const std::size_t targetBytes = std::size_t(512) << 20;
-    int64_t kc = std::max<int64_t>(256, static_cast<int64_t>(targetBytes / bytesPerK));
+    const std::size_t targetBytes = std::max<std::size_t>(1, ctx.stream_chunk_bytes());
+    int64_t kc = std::max<int64_t>(64, static_cast<int64_t>(targetBytes / bytesPerK));
     kc = std::min<int64_t>(kc, k);
     if (kc < k) kc = std::max<int64_t>(64, kc / 64 * 64);
     const int64_t numChunks = (k + kc - 1) / kc;
