@@ -7,13 +7,18 @@ grid, alpha = 1, beta = 0.5; A, B and C hold values uniform in [-1, 1) per
 component (splitmix64, seed 42, matrix ids 1/2/3) and are resident in HBM
 when the timed region starts.
 
+Forms: "plain" (FP64 MFMA kernels) and "split" (FP64-accurate split precision:
+Ozaki scheme II on the INT8 matrix cores, falling back to FP64 when its error
+bound cannot be met).
+
 value: FP64-accurate TFLOP/s (8*m*n*k / seconds per step) of the fastest form
 whose measured normwise error ||C - C_dd|| / (||A|| ||B||) (sampled entries
 against the double-double oracle) is <= 1e-13. Every form run is reported.
 
-roofline: the dominant kernel's average duration, from HIP events recorded on
-the stream the library launches on (the context runs on torch's current
-stream), against the FP64 matrix peak of MI355X (78.6 TFLOP/s).
+roofline: the dominant kernel of that form (FP64 GEMM kernel, or the batched
+INT8 GEMM of the split form), its duration from HIP events the library records
+on its launch stream (per-phase timing of the context), against MI355X's
+published peak (FP64 matrix 78.6 TFLOP/s; INT8 dense 5.0 POP/s).
 
 cpu_baseline: SPLA's CPU path (one host-BLAS zgemm, as in
 src/gemm/gemm_host.cpp:55-72) on a k-truncated sample of the same workload,
@@ -38,6 +43,7 @@ SEED = 42
 ALPHA = 1.0 + 0.0j
 BETA = 0.5 + 0.0j
 FP64_PEAK_TFLOPS = 78.6
+INT8_PEAK_TOPS = 5000.0
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
 
@@ -154,7 +160,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="plain", help="comma list of: plain")
+    ap.add_argument("--forms", default="plain,split", help="comma list of: plain, split")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -188,8 +194,11 @@ def main() -> int:
                    64, 0, 0, dist, ctx)
 
     flops = 8.0 * m * n * k
+    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8}
     forms = {}
     for form in args.forms.split(","):
+        ctx.set_gemm_mode(modes[form])
+        ctx.set_timing(True)
         dC = dC0.clone()
         for w in range(args.warmup):
             spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, dA, k, dB, k, BETA, dC,
@@ -199,22 +208,34 @@ def main() -> int:
         torch.cuda.synchronize()
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
         first_result = None
+        phase_sum = {}
         t0 = time.perf_counter()
         for s in range(args.steps):
             evs[2 * s].record(stream)
             spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, dA, k, dB, k, BETA, dC,
                            m, 0, 0, dist, ctx)
             evs[2 * s + 1].record(stream)
+            for name, v in ctx.last_timing().items():
+                phase_sum[name] = phase_sum.get(name, 0.0) + v
             if s == 0 and args.steps > 1:
                 first_result = dC.clone()
-            log(f"{form}: step {s + 1}/{args.steps} done")
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
+        log(f"{form}: {args.steps} steps done")
         kernel_ms = sum(evs[2 * s].elapsed_time(evs[2 * s + 1]) for s in range(args.steps))
         kernel_ms /= args.steps
+        info = ctx.last_product_info()
         res = first_result if first_result is not None else dC
-        entry = {"tflops": round(flops / wall / 1e12, 3), "ms_per_step": round(wall * 1e3, 2),
-                 "kernel_ms": round(kernel_ms, 2)}
+        entry = {"tflops": round(flops / wall / 1e12, 3), "ms_per_step": round(wall * 1e3, 3),
+                 "device_ms_per_step": round(kernel_ms, 3),
+                 "phase_ms_per_step": {p: round(v / args.steps, 4)
+                                       for p, v in phase_sum.items() if v > 0}}
+        if form == "split":
+            entry["split_used"] = info["split"]
+            entry["moduli"] = info["moduli"]
+            entry["bits"] = [info["bits_a"], info["bits_b"]]
+            entry["error_bound"] = info["bound"]
+            entry["fallback_reason"] = info["reason"]
         if not args.no_check:
             err, cnt = sampled_error(dA, dB, dC0, res, k, m, n)
             entry["normwise_error"] = err
@@ -222,11 +243,29 @@ def main() -> int:
             entry["accurate"] = err <= 1e-13
             log(f"{form}: {entry['tflops']} TFLOP/s, sampled normwise error {err:.2e}")
         forms[form] = entry
+        ctx.set_timing(False)
 
     ok = {f: e for f, e in forms.items() if e.get("accurate", True)}
     best_form = max(ok, key=lambda f: ok[f]["tflops"]) if ok else "plain"
     best = forms[best_form]
-    achieved = flops / (best["kernel_ms"] / 1e3) / 1e12
+    if best_form == "split" and best.get("split_used"):
+        # dominant kernel: the batched INT8 GEMM; executed ops = 2 m n kpad per product
+        kpad = (k + 127) // 128 * 128
+        ops = 2.0 * m * n * kpad * 3 * best["moduli"]
+        gemm_ms = best["phase_ms_per_step"].get("split_int8_gemm", float("nan"))
+        achieved = ops / (gemm_ms / 1e3) / 1e12
+        roofline = {"bound": "mfma", "kernel": "i8_gemm_kernel (v_mfma_i32_32x32x32_i8)",
+                    "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                    "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None}
+        dtype = "i8"
+    else:
+        gemm_ms = best["phase_ms_per_step"].get("gemm_fp64", best["device_ms_per_step"])
+        achieved = flops / (gemm_ms / 1e3) / 1e12
+        roofline = {"bound": "mfma", "kernel": "gemm_f64_kernel (v_mfma_f64_16x16x4_f64)",
+                    "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                    "traffic": None}
+        dtype = "f64"
     line = {
         "metric": METRIC,
         "value": best["tflops"],
@@ -238,15 +277,13 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic (splitmix64 seed 42, uniform [-1,1) per component), device-resident",
         "config": {"workload": "spla_pzgemm_ssb A^H B complex double, 1 rank (MPI_COMM_SELF)",
                    "m": m, "n": n, "k": k, "alpha": 1.0, "beta": 0.5,
                    "distC": "block cyclic 256x256 on 1x1", "form": best_form},
         "forms": forms,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None},
+        "roofline": roofline,
     }
     if not args.no_cpu_baseline:
         log("CPU baseline (host BLAS, k-truncated sample)")

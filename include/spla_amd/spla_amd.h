@@ -10,6 +10,22 @@
 #include "spla/context.h"
 #include "spla/errors.h"
 
+/* Arithmetic of double and complex-double products. */
+enum SplaAmdGemmMode {
+  /* Plain FP64 on the FP64 matrix cores (default, SPLA's semantics). */
+  SPLA_AMD_GEMM_FP64 = 0,
+  /* FP64-accurate split precision (Ozaki scheme II on the INT8 matrix cores)
+   * whenever the estimated normwise error bound meets the tolerance; FP64
+   * otherwise (non-finite input, too wide an exponent range, too many moduli). */
+  SPLA_AMD_GEMM_SPLIT_INT8 = 1,
+  /* Split precision only when it is also expected to be faster than FP64. */
+  SPLA_AMD_GEMM_AUTO = 2
+};
+
+#ifndef __cplusplus
+typedef enum SplaAmdGemmMode SplaAmdGemmMode;
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -28,6 +44,52 @@ SPLA_EXPORT SplaError spla_amd_ctx_set_stream(SplaContext ctx, void* hipStream);
  * Smaller chunks start the pipeline earlier, larger ones run longer kernels.
  */
 SPLA_EXPORT SplaError spla_amd_ctx_set_stream_chunk_bytes(SplaContext ctx, size_t bytes);
+
+/* Selects the arithmetic of double / complex-double products (see SplaAmdGemmMode). */
+SPLA_EXPORT SplaError spla_amd_ctx_set_gemm_mode(SplaContext ctx, SplaAmdGemmMode mode);
+
+/*
+ * Normwise error target ||C - C_exact||_F / (||A||_F ||B||_F) of the split
+ * precision form (default 1e-14; never looser than the FP64 bound k * 2^-53 of
+ * the same product). Must be positive.
+ */
+SPLA_EXPORT SplaError spla_amd_ctx_set_split_tolerance(SplaContext ctx, double tolerance);
+
+/*
+ * Outcome of the context's last double / complex-double product: used = 1 if
+ * it ran in split precision; reason = 0 when used, else why FP64 ran
+ * (1 non-finite input, 2 zero operand, 3 exponent range too wide, 4 too many
+ * moduli, 5 norm out of range, 6 FP64 expected faster, 7 split mode off);
+ * moduli, bitsA and bitsB describe the plan; bound is its estimated normwise
+ * error bound. Any output pointer may be NULL.
+ */
+SPLA_EXPORT SplaError spla_amd_ctx_last_product_info(SplaContext ctx, int* used, int* reason,
+                                                     int* moduli, int* bitsA, int* bitsB,
+                                                     double* bound);
+
+/* Kernel phases reported by spla_amd_ctx_last_timing. */
+enum SplaAmdPhase {
+  SPLA_AMD_PHASE_GEMM_FP64 = 0,        /* FP64 MFMA GEMM kernels */
+  SPLA_AMD_PHASE_SPLITK_REDUCE = 1,    /* split-K reduction and alpha / beta update */
+  SPLA_AMD_PHASE_SPLIT_STATS = 2,      /* split precision: operand statistics */
+  SPLA_AMD_PHASE_SPLIT_RESIDUES = 3,   /* split precision: residue planes */
+  SPLA_AMD_PHASE_SPLIT_INT8_GEMM = 4,  /* split precision: batched INT8 GEMM */
+  SPLA_AMD_PHASE_SPLIT_CRT = 5,        /* split precision: reconstruction and update */
+  SPLA_AMD_NUM_PHASES = 6
+};
+
+/*
+ * Enables (1) or disables (0) per-phase kernel timing: HIP events are recorded
+ * on the launch stream around every phase of the following calls.
+ */
+SPLA_EXPORT SplaError spla_amd_ctx_set_timing(SplaContext ctx, int enable);
+
+/*
+ * Milliseconds spent by each SplaAmdPhase in the context's last call (summed
+ * over that call's launches; 0 for phases that did not run). ms must hold
+ * SPLA_AMD_NUM_PHASES doubles.
+ */
+SPLA_EXPORT SplaError spla_amd_ctx_last_timing(SplaContext ctx, double* ms);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@ from spla_amd.api import (  # noqa: F401
     Context,
     DistributionType,
     FillMode,
+    GemmMode,
     GenericError,
     GPUAllocationError,
     GPUError,

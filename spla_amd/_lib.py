@@ -82,6 +82,12 @@ def _declare(lib: ctypes.CDLL) -> None:
     # include/spla_amd/spla_amd.h
     fn("spla_amd_ctx_set_stream", c_void_p, c_void_p)
     fn("spla_amd_ctx_set_stream_chunk_bytes", c_void_p, ctypes.c_size_t)
+    fn("spla_amd_ctx_set_gemm_mode", c_void_p, c_int)
+    fn("spla_amd_ctx_set_split_tolerance", c_void_p, c_double)
+    fn("spla_amd_ctx_set_timing", c_void_p, c_int)
+    fn("spla_amd_ctx_last_timing", c_void_p, P(c_double))
+    fn("spla_amd_ctx_last_product_info", c_void_p, P(c_int), P(c_int), P(c_int), P(c_int),
+       P(c_int), P(c_double))
 
 
 def lib() -> ctypes.CDLL:

@@ -36,6 +36,13 @@ class FillMode(enum.IntEnum):
     LOWER = 2
 
 
+class GemmMode(enum.IntEnum):
+    """Arithmetic of double / complex-double products (include/spla_amd/spla_amd.h)."""
+    FP64 = 0
+    SPLIT_INT8 = 1
+    AUTO = 2
+
+
 class DistributionType(enum.IntEnum):
     BLACS_BLOCK_CYCLIC = 0
     MIRROR = 1
@@ -198,6 +205,33 @@ class Context:
         """Runs the kernels on a caller stream (torch.cuda.Stream, raw handle or None)."""
         handle = getattr(stream, "cuda_stream", stream)
         check(self._lib.spla_amd_ctx_set_stream(self._h, handle or None))
+
+    def set_gemm_mode(self, mode: GemmMode) -> None:
+        check(self._lib.spla_amd_ctx_set_gemm_mode(self._h, int(mode)))
+
+    def set_split_tolerance(self, tol: float) -> None:
+        check(self._lib.spla_amd_ctx_set_split_tolerance(self._h, float(tol)))
+
+    def last_product_info(self) -> dict:
+        """Form of the last double / complex-double product (split plan, error bound)."""
+        v = [ctypes.c_int() for _ in range(5)]
+        b = ctypes.c_double()
+        check(self._lib.spla_amd_ctx_last_product_info(self._h, *[ctypes.byref(x) for x in v],
+                                                       ctypes.byref(b)))
+        return {"split": bool(v[0].value), "reason": v[1].value, "moduli": v[2].value,
+                "bits_a": v[3].value, "bits_b": v[4].value, "bound": b.value}
+
+    PHASES = ("gemm_fp64", "splitk_reduce", "split_stats", "split_residues", "split_int8_gemm",
+              "split_crt")
+
+    def set_timing(self, enable: bool) -> None:
+        check(self._lib.spla_amd_ctx_set_timing(self._h, 1 if enable else 0))
+
+    def last_timing(self) -> dict:
+        """Milliseconds per kernel phase of the last call (timing must be enabled)."""
+        ms = (ctypes.c_double * len(self.PHASES))()
+        check(self._lib.spla_amd_ctx_last_timing(self._h, ms))
+        return dict(zip(self.PHASES, list(ms)))
 
     def set_stream_chunk_bytes(self, nbytes: int) -> None:
         """Target bytes of one streamed k-chunk of host-resident operands."""

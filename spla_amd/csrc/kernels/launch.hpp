@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "kernels/gemm_desc.hpp"
+#include "kernels/ozaki_desc.hpp"
 
 namespace spla_amd {
 
@@ -26,5 +27,15 @@ hipError_t launch_convert_f32_f64(const float* src, int64_t srcLd, double* dst, 
                                   int64_t rows, int64_t cols, hipStream_t stream);
 hipError_t launch_convert_f64_f32(const double* src, int64_t srcLd, float* dst, int64_t dstLd,
                                   int64_t rows, int64_t cols, hipStream_t stream);
+
+// Split-precision kernels (ozaki.hip).
+// Per-row largest |component| and sum of squares of an operand read as `rows`
+// rows of k elements; `partial` holds 2 * splits * rows doubles.
+hipError_t launch_split_stats(bool complex, int layout, const double* X, int64_t ld, int rows,
+                              int64_t k, double* partial, int splits, double* outMax,
+                              double* outSum, hipStream_t stream);
+hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipStream_t stream);
+hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream);
+hipError_t launch_crt(bool complex, const CrtDesc& d, hipStream_t stream);
 
 }  // namespace spla_amd

@@ -1,0 +1,657 @@
+// Split-precision (Ozaki scheme II) kernels for gfx950: FP64-accurate products
+// on the INT8 matrix cores (v_mfma_i32_32x32x32_i8). See ozaki_desc.hpp for
+// the arithmetic.
+//
+//   stats_*      per-row largest |component|, sum of squares, non-finite flag
+//                (HBM bound: reads each operand once);
+//   residue_*    scaled integer rows -> symmetric residues modulo every m_t,
+//                written as K-major int8 planes padded to 128 bytes
+//                (HBM bound: reads the operand once, writes N*products bytes
+//                per element);
+//   i8_gemm      batched C_g = R_A,g R_B,g^T over all moduli and complex
+//                sub-products in one launch, INT32 accumulation; splits of at
+//                most 2^17 - 128 reduction bytes keep the sums below 2^31
+//                (|residue| <= 128). One split: the epilogue reduces modulo
+//                m_t and stores uint8; several splits: raw int32 sums;
+//   crt          per output element: modular recombination of the complex
+//                sub-products, Chinese remainder reconstruction in 192-bit
+//                integer arithmetic, conversion to FP64, scaling by
+//                2^-(ea_i + eb_j) and the alpha / beta update.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels/launch.hpp"
+#include "kernels/ozaki_desc.hpp"
+
+namespace spla_amd {
+namespace {
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef int int16v __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// statistics
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) stats_kmajor_kernel(const double* __restrict__ X,
+                                                           int64_t ld, int rows, int64_t Kp,
+                                                           int64_t perSplit, double* pmax,
+                                                           double* psum) {
+  const int r = blockIdx.x;
+  const int s = blockIdx.y;
+  const int64_t q0 = static_cast<int64_t>(s) * perSplit;
+  const int64_t q1 = min(Kp, q0 + perSplit);
+  const double* row = X + static_cast<int64_t>(r) * ld * (CPLX ? 2 : 1);
+  double mx = 0.0, sq = 0.0;
+  int bad = 0;
+  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
+    const double x = row[q];
+    bad |= !isfinite(x);
+    mx = fmax(mx, fabs(x));
+    sq += x * x;
+  }
+  __shared__ double smx[256], ssq[256];
+  __shared__ int sbad[256];
+  smx[threadIdx.x] = mx;
+  ssq[threadIdx.x] = sq;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + w]);
+      ssq[threadIdx.x] += ssq[threadIdx.x + w];
+      sbad[threadIdx.x] |= sbad[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmax[static_cast<int64_t>(s) * rows + r] = sbad[0] ? __builtin_nan("") : smx[0];
+    psum[static_cast<int64_t>(s) * rows + r] = ssq[0];
+  }
+}
+
+// Rows contiguous in memory: 64 rows per block (one per lane), 4 waves split the
+// reduction index.
+template <bool CPLX>
+__global__ void __launch_bounds__(256) stats_mnmajor_kernel(const double* __restrict__ X,
+                                                            int64_t ld, int rows, int64_t k,
+                                                            int64_t perSplit, double* pmax,
+                                                            double* psum) {
+  const int tx = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + tx;
+  const int s = blockIdx.y;
+  const int64_t l0 = static_cast<int64_t>(s) * perSplit;
+  const int64_t l1 = min(k, l0 + perSplit);
+  double mx = 0.0, sq = 0.0;
+  int bad = 0;
+  if (r < rows) {
+    for (int64_t l = l0 + ty; l < l1; l += 4) {
+      if (CPLX) {
+        const double* p = X + (l * ld + r) * 2;
+        const double a = p[0], b = p[1];
+        bad |= !isfinite(a) | !isfinite(b);
+        mx = fmax(mx, fmax(fabs(a), fabs(b)));
+        sq += a * a + b * b;
+      } else {
+        const double a = X[l * ld + r];
+        bad |= !isfinite(a);
+        mx = fmax(mx, fabs(a));
+        sq += a * a;
+      }
+    }
+  }
+  __shared__ double smx[4][64], ssq[4][64];
+  __shared__ int sbad[4][64];
+  smx[ty][tx] = mx;
+  ssq[ty][tx] = sq;
+  sbad[ty][tx] = bad;
+  __syncthreads();
+  if (ty == 0 && r < rows) {
+    for (int w = 1; w < 4; ++w) {
+      mx = fmax(mx, smx[w][tx]);
+      sq += ssq[w][tx];
+      bad |= sbad[w][tx];
+    }
+    pmax[static_cast<int64_t>(s) * rows + r] = bad ? __builtin_nan("") : mx;
+    psum[static_cast<int64_t>(s) * rows + r] = sq;
+  }
+}
+
+__global__ void __launch_bounds__(256) stats_finalize_kernel(const double* pmax,
+                                                             const double* psum, int rows,
+                                                             int splits, double* omax,
+                                                             double* osum) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= rows) return;
+  double mx = 0.0, sq = 0.0;
+  int bad = 0;
+  for (int s = 0; s < splits; ++s) {
+    const double v = pmax[static_cast<int64_t>(s) * rows + r];
+    bad |= isnan(v);
+    mx = fmax(mx, v);
+    sq += psum[static_cast<int64_t>(s) * rows + r];
+  }
+  omax[r] = bad ? __builtin_nan("") : mx;
+  osum[r] = sq;
+}
+
+// ---------------------------------------------------------------------------
+// residues
+
+// Symmetric residue of an exact integer a (|a| <= 2^54) modulo m:
+// [-(m/2), m - 1 - (m/2)], i.e. [-128, 127] for m = 256 and [-(m-1)/2, (m-1)/2] for odd m.
+__device__ __forceinline__ int sym_mod(double a, double m, double inv, double lo, double hi) {
+  const double q = rint(a * inv);
+  double r = fma(-q, m, a);  // exact: the true value is a small integer
+  r = (r > hi) ? r - m : r;
+  r = (r < lo) ? r + m : r;
+  return static_cast<int>(r);
+}
+
+__device__ __forceinline__ int sym_fold(int r, int m, int lo, int hi) {
+  r = (r > hi) ? r - m : r;
+  return (r < lo) ? r + m : r;
+}
+
+__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
+  return (static_cast<uint32_t>(a) & 0xffu) | ((static_cast<uint32_t>(b) & 0xffu) << 8) |
+         ((static_cast<uint32_t>(c) & 0xffu) << 16) | ((static_cast<uint32_t>(d) & 0xffu) << 24);
+}
+
+// Reduction index contiguous: one thread converts 16 consecutive elements of a row.
+template <bool CPLX>
+__global__ void __launch_bounds__(256) residue_kmajor_kernel(ResidueDesc d) {
+  const int row = blockIdx.y;
+  const int64_t k0 = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) * 16;
+  if (row >= d.rows || k0 >= d.kpad) return;
+  const int e = d.expo[row];
+  const double* src = d.X + static_cast<int64_t>(row) * d.ld * (CPLX ? 2 : 1);
+  double re[16], im[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int64_t l = k0 + j;
+    re[j] = 0.0;
+    im[j] = 0.0;
+    if (l < d.k) {
+      if (CPLX) {
+        re[j] = rint(ldexp(src[2 * l], e));
+        im[j] = d.imagSign * rint(ldexp(src[2 * l + 1], e));
+      } else {
+        re[j] = rint(ldexp(src[l], e));
+      }
+    }
+  }
+  constexpr int P = CPLX ? 3 : 1;
+  for (int t = 0; t < d.ms.count; ++t) {
+    const int mi = d.ms.m[t];
+    const double m = mi, inv = d.ms.inv[t];
+    const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
+    int r0[16], r1[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      r0[j] = sym_mod(re[j], m, inv, lo, hi);
+      r1[j] = CPLX ? sym_mod(im[j], m, inv, lo, hi) : 0;
+    }
+    int8_t* base = d.R + static_cast<int64_t>(t * P) * d.plane + static_cast<int64_t>(row) * d.kpad + k0;
+    int4v w0;
+    w0[0] = pack4(r0[0], r0[1], r0[2], r0[3]);
+    w0[1] = pack4(r0[4], r0[5], r0[6], r0[7]);
+    w0[2] = pack4(r0[8], r0[9], r0[10], r0[11]);
+    w0[3] = pack4(r0[12], r0[13], r0[14], r0[15]);
+    *reinterpret_cast<int4v*>(base) = w0;
+    if (CPLX) {
+      int4v w1, w2;
+      int r2[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) r2[j] = sym_fold(r0[j] + r1[j], mi, lo, hi);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        w1[q] = pack4(r1[4 * q], r1[4 * q + 1], r1[4 * q + 2], r1[4 * q + 3]);
+        w2[q] = pack4(r2[4 * q], r2[4 * q + 1], r2[4 * q + 2], r2[4 * q + 3]);
+      }
+      *reinterpret_cast<int4v*>(base + d.plane) = w1;
+      *reinterpret_cast<int4v*>(base + 2 * d.plane) = w2;
+    }
+  }
+}
+
+// Rows contiguous: a 64 x 64 tile is read along rows and transposed through LDS
+// so that the planes are written along the reduction index.
+template <bool CPLX>
+__global__ void __launch_bounds__(256) residue_mnmajor_kernel(ResidueDesc d) {
+  constexpr int P = CPLX ? 3 : 1;
+  constexpr int kStride = 64 + 16;
+  __shared__ __attribute__((aligned(16))) int8_t lds[P][64][kStride];
+  const int tx = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const int row0 = blockIdx.y * 64;
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64;
+  const int row = row0 + tx;
+  const int e = row < d.rows ? d.expo[row] : 0;
+  double re[16], im[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int64_t l = k0 + ty + 4 * j;
+    re[j] = 0.0;
+    im[j] = 0.0;
+    if (row < d.rows && l < d.k) {
+      if (CPLX) {
+        const double* p = d.X + (l * d.ld + row) * 2;
+        re[j] = rint(ldexp(p[0], e));
+        im[j] = d.imagSign * rint(ldexp(p[1], e));
+      } else {
+        re[j] = rint(ldexp(d.X[l * d.ld + row], e));
+      }
+    }
+  }
+  const int wr = threadIdx.x >> 2;
+  const int ws = threadIdx.x & 3;
+  for (int t = 0; t < d.ms.count; ++t) {
+    const int mi = d.ms.m[t];
+    const double m = mi, inv = d.ms.inv[t];
+    const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r0 = sym_mod(re[j], m, inv, lo, hi);
+      lds[0][tx][ty + 4 * j] = static_cast<int8_t>(r0);
+      if (CPLX) {
+        const int r1 = sym_mod(im[j], m, inv, lo, hi);
+        lds[1 % P][tx][ty + 4 * j] = static_cast<int8_t>(r1);
+        lds[2 % P][tx][ty + 4 * j] = static_cast<int8_t>(sym_fold(r0 + r1, mi, lo, hi));
+      }
+    }
+    __syncthreads();
+    if (row0 + wr < d.rows) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        int8_t* dst = d.R + static_cast<int64_t>(t * P + p) * d.plane +
+                      static_cast<int64_t>(row0 + wr) * d.kpad + k0 + 16 * ws;
+        *reinterpret_cast<int4v*>(dst) = *reinterpret_cast<const int4v*>(&lds[p][wr][16 * ws]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched INT8 GEMM
+
+__device__ __forceinline__ bool tile_active(int mode, int64_t rowOff, int64_t colOff, int64_t r,
+                                            int64_t c, int64_t nr, int64_t nc) {
+  // SPLA block_is_active (src/block_generation/block.hpp:58-66) on a tile.
+  if (mode == 1) return c + nc + colOff > r + rowOff;
+  if (mode == 2) return r + nr + rowOff > c + colOff;
+  return true;
+}
+
+template <bool OUT8, int WM, int WN>
+__global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
+  constexpr int BM = 2 * WM;
+  constexpr int BN = 2 * WN;
+  constexpr int BK = 128;
+  constexpr int kStride = BK + 16;
+  constexpr int FM = WM / 32;
+  constexpr int FN = WN / 32;
+  constexpr int CA = BM * (BK / 16) / 256;  // 16-byte chunks per thread
+  constexpr int CB = BN * (BK / 16) / 256;
+  __shared__ __attribute__((aligned(16))) int8_t lA[2][BM][kStride];
+  __shared__ __attribute__((aligned(16))) int8_t lB[2][BN][kStride];
+
+  // block -> (split, product, tile); consecutive blocks land on one XCD after the remap
+  const int G = gridDim.x;
+  const int bid = blockIdx.x;
+  int lin = bid;
+  if ((G & 7) == 0) lin = (bid & 7) * (G >> 3) + (bid >> 3);
+  const int tiles = d.tilesM * d.tilesN;
+  const int s = lin / (d.G * tiles);
+  int rest = lin - s * d.G * tiles;
+  const int g = rest / tiles;
+  const int t = rest - g * tiles;
+  const int group = t / (8 * d.tilesN);
+  const int firstM = group * 8;
+  const int gsize = min(d.tilesM - firstM, 8);
+  const int inGroup = t - group * 8 * d.tilesN;
+  const int tm = firstM + inGroup % gsize;
+  const int tn = inGroup / gsize;
+  const int row0 = tm * BM;
+  const int col0 = tn * BN;
+  if (d.fillMode != 0 &&
+      !tile_active(d.fillMode, d.fillRowOff, d.fillColOff, row0, col0, min(BM, d.m - row0),
+                   min(BN, d.n - col0)))
+    return;
+
+  const int64_t kBegin = static_cast<int64_t>(s) * d.kSplit;
+  const int64_t kEnd = min(d.kpad, kBegin + d.kSplit);
+  const int nk = static_cast<int>((kEnd - kBegin) / BK);
+  const int8_t* Ab = d.RA + static_cast<int64_t>(g) * d.planeA;
+  const int8_t* Bb = d.RB + static_cast<int64_t>(g) * d.planeB;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const int wm = wid & 1;
+  const int wn = wid >> 1;
+  const int l32 = lane & 31;
+  const int khalf = lane >> 5;
+
+  int16v acc[FM][FN];
+#pragma unroll
+  for (int f = 0; f < FM; ++f)
+#pragma unroll
+    for (int q = 0; q < FN; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[f][q][r] = 0;
+
+  int4v ra[CA], rb[CB];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int u = 0; u < CA; ++u) {
+      const int c = tid + u * 256;
+      const int row = c >> 3, seg = c & 7;
+      ra[u] = int4v{0, 0, 0, 0};
+      if (row0 + row < d.m)
+        ra[u] = *reinterpret_cast<const int4v*>(Ab + static_cast<int64_t>(row0 + row) * d.kpad +
+                                                k0 + 16 * seg);
+    }
+#pragma unroll
+    for (int u = 0; u < CB; ++u) {
+      const int c = tid + u * 256;
+      const int row = c >> 3, seg = c & 7;
+      rb[u] = int4v{0, 0, 0, 0};
+      if (col0 + row < d.n)
+        rb[u] = *reinterpret_cast<const int4v*>(Bb + static_cast<int64_t>(col0 + row) * d.kpad +
+                                                k0 + 16 * seg);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < CA; ++u) {
+      const int c = tid + u * 256;
+      *reinterpret_cast<int4v*>(&lA[buf][c >> 3][16 * (c & 7)]) = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < CB; ++u) {
+      const int c = tid + u * 256;
+      *reinterpret_cast<int4v*>(&lB[buf][c >> 3][16 * (c & 7)]) = rb[u];
+    }
+  };
+
+  if (nk > 0) {
+    load(kBegin);
+    store(0);
+  }
+  __syncthreads();
+  for (int it = 0; it < nk; ++it) {
+    const int cur = it & 1;
+    const bool more = it + 1 < nk;
+    if (more) load(kBegin + static_cast<int64_t>(it + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK / 32; ++ks) {
+      int4v a[FM], b[FN];
+#pragma unroll
+      for (int f = 0; f < FM; ++f)
+        a[f] = *reinterpret_cast<const int4v*>(&lA[cur][wm * WM + 32 * f + l32][ks * 32 + 16 * khalf]);
+#pragma unroll
+      for (int q = 0; q < FN; ++q)
+        b[q] = *reinterpret_cast<const int4v*>(&lB[cur][wn * WN + 32 * q + l32][ks * 32 + 16 * khalf]);
+#pragma unroll
+      for (int f = 0; f < FM; ++f)
+#pragma unroll
+        for (int q = 0; q < FN; ++q)
+          acc[f][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[f], b[q], acc[f][q], 0, 0, 0);
+    }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  // D layout of v_mfma_i32_32x32x32_i8: lane holds column lane % 32 and rows
+  // 8 * (r / 4) + 4 * (lane / 32) + r % 4, r = 0..15.
+  const int mod = d.ms.m[g / d.products];
+  const double dm = mod, dinv = d.ms.inv[g / d.products];
+#pragma unroll
+  for (int f = 0; f < FM; ++f) {
+#pragma unroll
+    for (int q = 0; q < FN; ++q) {
+      const int col = col0 + wn * WN + 32 * q + l32;
+      if (col >= d.n) continue;
+#pragma unroll
+      for (int rb4 = 0; rb4 < 4; ++rb4) {
+        const int row = row0 + wm * WM + 32 * f + 8 * rb4 + 4 * khalf;
+        if (OUT8) {
+          uint8_t* dst = static_cast<uint8_t*>(d.out) + static_cast<int64_t>(g) * d.outPlane +
+                         static_cast<int64_t>(col) * d.m + row;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (row + r < d.m) {
+              const double v = acc[f][q][4 * rb4 + r];
+              double rr = v - dm * floor(v * dinv);
+              rr = rr < 0.0 ? rr + dm : (rr >= dm ? rr - dm : rr);
+              dst[r] = static_cast<uint8_t>(static_cast<int>(rr));
+            }
+          }
+        } else {
+          int* dst = static_cast<int*>(d.out) +
+                     (static_cast<int64_t>(s) * d.G + g) * d.outPlane +
+                     static_cast<int64_t>(col) * d.m + row;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (row + r < d.m) dst[r] = acc[f][q][4 * rb4 + r];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Chinese remainder reconstruction and FP64 epilogue
+
+struct U192 {
+  uint64_t w0, w1, w2;
+};
+
+__device__ __forceinline__ void mac(U192& x, const uint64_t* W, uint64_t r) {
+  const uint64_t lo0 = W[0] * r, hi0 = __umul64hi(W[0], r);
+  const uint64_t lo1 = W[1] * r, hi1 = __umul64hi(W[1], r);
+  const uint64_t lo2 = W[2] * r;
+  x.w0 += lo0;
+  uint64_t c = x.w0 < lo0;
+  uint64_t t = lo1 + hi0;
+  uint64_t c1 = t < lo1;
+  t += c;
+  c1 += t < c;
+  x.w1 += t;
+  c1 += x.w1 < t;
+  x.w2 += lo2 + hi1 + c1;
+}
+
+__device__ __forceinline__ bool geq(const U192& a, const uint64_t* b) {
+  if (a.w2 != b[2]) return a.w2 > b[2];
+  if (a.w1 != b[1]) return a.w1 > b[1];
+  return a.w0 >= b[0];
+}
+
+__device__ __forceinline__ void sub_in(U192& a, const U192& b) {  // a -= b, a >= b
+  const uint64_t b0 = a.w0 < b.w0;
+  a.w0 -= b.w0;
+  const uint64_t t1 = b.w1 + b0;
+  const uint64_t b1 = (a.w1 < t1) | (t1 < b0);
+  a.w1 -= t1;
+  a.w2 -= b.w2 + b1;
+}
+
+__device__ __forceinline__ double to_double(const U192& x) {
+  return ldexp(static_cast<double>(x.w2), 128) + ldexp(static_cast<double>(x.w1), 64) +
+         static_cast<double>(x.w0);
+}
+
+// x mod M, then mapped to the symmetric range and converted to double.
+__device__ __forceinline__ double crt_finish(U192 x, const CrtConstants& cc) {
+  const double q = floor(to_double(x) / cc.Md);
+  U192 M{cc.M[0], cc.M[1], cc.M[2]};
+  if (q > 0.0) {
+    // subtract (q - 1) * M to stay non-negative, then fix up by comparison
+    const uint64_t qi = static_cast<uint64_t>(q) - 1;
+    U192 qm{0, 0, 0};
+    mac(qm, cc.M, qi);
+    if (geq(x, &qm.w0)) sub_in(x, qm);
+  }
+  for (int it = 0; it < 3 && geq(x, cc.M); ++it) sub_in(x, M);
+  if (geq(x, cc.halfM) && !(x.w0 == cc.halfM[0] && x.w1 == cc.halfM[1] && x.w2 == cc.halfM[2])) {
+    U192 neg = M;
+    sub_in(neg, x);
+    return -to_double(neg);
+  }
+  return to_double(x);
+}
+
+__device__ __forceinline__ uint64_t umod(int64_t v, int m, double inv) {
+  const double vd = static_cast<double>(v);  // |v| < 2^53
+  double r = vd - static_cast<double>(m) * floor(vd * inv);
+  r = r < 0.0 ? r + m : (r >= m ? r - m : r);
+  return static_cast<uint64_t>(r);
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t total = static_cast<int64_t>(d.m) * d.n;
+  if (idx >= total) return;
+  const int i = static_cast<int>(idx % d.m);
+  const int j = static_cast<int>(idx / d.m);
+  if (d.fillMode != 0) {
+    const int64_t r0 = (i / d.tileRows) * static_cast<int64_t>(d.tileRows);
+    const int64_t c0 = (j / d.tileCols) * static_cast<int64_t>(d.tileCols);
+    if (!tile_active(d.fillMode, d.fillRowOff, d.fillColOff, r0, c0,
+                     min<int64_t>(d.tileRows, d.m - r0), min<int64_t>(d.tileCols, d.n - c0)))
+      return;
+  }
+  const CrtConstants& cc = *d.cc;
+  U192 xr{0, 0, 0}, xi{0, 0, 0};
+  for (int t = 0; t < d.ms.count; ++t) {
+    const int m = d.ms.m[t];
+    const double inv = d.ms.inv[t];
+    int64_t T[3] = {0, 0, 0};
+#pragma unroll
+    for (int p = 0; p < (CPLX ? 3 : 1); ++p) {
+      const int g = t * d.products + p;
+      if (d.in8) {
+        T[p] = static_cast<const uint8_t*>(d.W)[static_cast<int64_t>(g) * d.plane + idx];
+      } else {
+        const int* w = static_cast<const int*>(d.W);
+        int64_t sum = 0;
+        for (int s = 0; s < d.splits; ++s)
+          sum += w[(static_cast<int64_t>(s) * d.G + g) * d.plane + idx];
+        T[p] = sum;
+      }
+    }
+    if (CPLX) {
+      mac(xr, cc.W[t], umod(T[0] - T[1], m, inv));
+      mac(xi, cc.W[t], umod(T[2] - T[0] - T[1], m, inv));
+    } else {
+      mac(xr, cc.W[t], umod(T[0], m, inv));
+    }
+  }
+  const int sc = -(d.expA[i] + d.expB[j]);
+  const double re = ldexp(crt_finish(xr, cc), sc);
+  const double im = CPLX ? ldexp(crt_finish(xi, cc), sc) : 0.0;
+  if (CPLX) {
+    double* c = d.C + (static_cast<int64_t>(j) * d.ldc + i) * 2;
+    double outRe = d.alphaRe * re - d.alphaIm * im;
+    double outIm = d.alphaRe * im + d.alphaIm * re;
+    if (!d.betaZero) {
+      const double cr = c[0], ci = c[1];
+      outRe += d.betaRe * cr - d.betaIm * ci;
+      outIm += d.betaRe * ci + d.betaIm * cr;
+    }
+    c[0] = outRe;
+    c[1] = outIm;
+  } else {
+    double* c = d.C + static_cast<int64_t>(j) * d.ldc + i;
+    double out = d.alphaRe * re;
+    if (!d.betaZero) out += d.betaRe * c[0];
+    c[0] = out;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_split_stats(bool complex, int layout, const double* X, int64_t ld, int rows,
+                              int64_t k, double* partial, int splits, double* outMax,
+                              double* outSum, hipStream_t stream) {
+  if (rows == 0) return hipSuccess;
+  double* pmax = partial;
+  double* psum = partial + static_cast<int64_t>(splits) * rows;
+  if (layout == KMajor) {
+    const int64_t Kp = complex ? 2 * k : k;
+    const int64_t per = (Kp + splits - 1) / splits;
+    const dim3 grid(rows, splits);
+    if (complex)
+      hipLaunchKernelGGL(stats_kmajor_kernel<true>, grid, dim3(256), 0, stream, X, ld, rows, Kp,
+                         per, pmax, psum);
+    else
+      hipLaunchKernelGGL(stats_kmajor_kernel<false>, grid, dim3(256), 0, stream, X, ld, rows, Kp,
+                         per, pmax, psum);
+  } else {
+    const int64_t per = (k + splits - 1) / splits;
+    const dim3 grid((rows + 63) / 64, splits);
+    if (complex)
+      hipLaunchKernelGGL(stats_mnmajor_kernel<true>, grid, dim3(256), 0, stream, X, ld, rows, k,
+                         per, pmax, psum);
+    else
+      hipLaunchKernelGGL(stats_mnmajor_kernel<false>, grid, dim3(256), 0, stream, X, ld, rows, k,
+                         per, pmax, psum);
+  }
+  hipLaunchKernelGGL(stats_finalize_kernel, dim3((rows + 255) / 256), dim3(256), 0, stream, pmax,
+                     psum, rows, splits, outMax, outSum);
+  return hipGetLastError();
+}
+
+hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipStream_t stream) {
+  if (d.rows == 0) return hipSuccess;
+  if (layout == KMajor) {
+    const dim3 grid(static_cast<unsigned>((d.kpad / 16 + 255) / 256), d.rows);
+    if (complex)
+      hipLaunchKernelGGL(residue_kmajor_kernel<true>, grid, dim3(256), 0, stream, d);
+    else
+      hipLaunchKernelGGL(residue_kmajor_kernel<false>, grid, dim3(256), 0, stream, d);
+  } else {
+    const dim3 grid(static_cast<unsigned>(d.kpad / 64), (d.rows + 63) / 64);
+    if (complex)
+      hipLaunchKernelGGL(residue_mnmajor_kernel<true>, grid, dim3(256), 0, stream, d);
+    else
+      hipLaunchKernelGGL(residue_mnmajor_kernel<false>, grid, dim3(256), 0, stream, d);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream) {
+  const unsigned blocks = static_cast<unsigned>(d.tilesM) * d.tilesN * d.G * d.splits;
+  const bool out8 = d.splits == 1;
+  if (cfg == I8TileM) {
+    if (out8)
+      hipLaunchKernelGGL((i8_gemm_kernel<true, 64, 64>), dim3(blocks), dim3(256), 0, stream, d);
+    else
+      hipLaunchKernelGGL((i8_gemm_kernel<false, 64, 64>), dim3(blocks), dim3(256), 0, stream, d);
+  } else {
+    if (out8)
+      hipLaunchKernelGGL((i8_gemm_kernel<true, 128, 128>), dim3(blocks), dim3(256), 0, stream, d);
+    else
+      hipLaunchKernelGGL((i8_gemm_kernel<false, 128, 128>), dim3(blocks), dim3(256), 0, stream, d);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_crt(bool complex, const CrtDesc& d, hipStream_t stream) {
+  const int64_t total = static_cast<int64_t>(d.m) * d.n;
+  if (total == 0) return hipSuccess;
+  const dim3 grid(static_cast<unsigned>((total + 255) / 256));
+  if (complex)
+    hipLaunchKernelGGL(crt_kernel<true>, grid, dim3(256), 0, stream, d);
+  else
+    hipLaunchKernelGGL(crt_kernel<false>, grid, dim3(256), 0, stream, d);
+  return hipGetLastError();
+}
+
+}  // namespace spla_amd

@@ -15,6 +15,8 @@
 #include <tuple>
 
 #include "runtime/allocators.hpp"
+#include "runtime/phase_timer.hpp"
+#include "runtime/split_report.hpp"
 #include "runtime/tuner.hpp"
 #include "spla/types.h"
 
@@ -64,6 +66,16 @@ public:
 
   spla_amd::Tuner& tuner() { return tuner_; }
 
+  // Arithmetic of FP64 / complex-FP64 products (include/spla_amd/spla_amd.h).
+  int gemm_mode() const { return gemmMode_; }
+  void set_gemm_mode(int m) { gemmMode_ = m; }
+  double split_tolerance() const { return splitTolerance_; }
+  void set_split_tolerance(double t) { splitTolerance_ = t; }
+  int split_max_moduli() const { return splitMaxModuli_; }
+  void set_split_max_moduli(int v) { splitMaxModuli_ = v; }
+  spla_amd::SplitReport& last_split() { return lastSplit_; }
+  spla_amd::PhaseTimer& timer() { return timer_; }
+
 private:
   void ensure_device_resources();
 
@@ -86,6 +98,11 @@ private:
   int numCUs_ = 0;
 
   spla_amd::Tuner tuner_;
+  int gemmMode_ = 0;
+  double splitTolerance_ = 1e-14;
+  int splitMaxModuli_ = 20;
+  spla_amd::SplitReport lastSplit_;
+  spla_amd::PhaseTimer timer_;
 };
 
 }  // namespace spla

@@ -11,6 +11,7 @@
 
 #include "kernels/launch.hpp"
 #include "runtime/hip_check.hpp"
+#include "runtime/split_gemm.hpp"
 #include "spla/exceptions.hpp"
 
 namespace spla_amd {
@@ -137,6 +138,7 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
   d.fillColOff = fill.colOff;
   d.splits = 1;
 
+  ctx.last_split() = SplitReport();
   if (k == 0 || (d.alphaRe == 0.0 && d.alphaIm == 0.0)) {
     // BLAS semantics: C <- beta * C, A and B are not read.
     hip_check(launch_scale(d, cplx, stream));
@@ -146,6 +148,10 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
   const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
   const GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus());
+  if (ctx.gemm_mode() != GemmModeFP64 &&
+      enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
+                               plan.estSeconds, stream, keep))
+    return;
   d.splits = plan.splits;
   d.kSplit = plan.kSplit;
   d.tilesM = plan.tilesM;
@@ -165,8 +171,14 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
       d.work = keep.back().get<double>();
     }
   }
-  hip_check(launch_gemm_f64(d, cplx, la, lb, plan.cfg, stream));
-  if (plan.splits > 1) hip_check(launch_splitk_reduce(d, cplx, plan.cfg, stream));
+  {
+    PhaseScope ps(ctx.timer(), PhaseGemmF64, stream);
+    hip_check(launch_gemm_f64(d, cplx, la, lb, plan.cfg, stream));
+  }
+  if (plan.splits > 1) {
+    PhaseScope ps(ctx.timer(), PhaseSplitKReduce, stream);
+    hip_check(launch_splitk_reduce(d, cplx, plan.cfg, stream));
+  }
 }
 
 // Copies `cols` columns of `rowBytes` bytes (source pitch srcPitch) into a
@@ -426,6 +438,13 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
   DeviceGuard guard(ctx.device());
   // SPLA synchronizes with the legacy default stream on entry (gemm_gpu.cpp:72).
   hip_check(hipStreamSynchronize(nullptr));
+  ctx.timer().reset();
+  struct Resolve {
+    spla::ContextInternal& c;
+    ~Resolve() {
+      if (c.timer().enabled()) c.timer().resolve();
+    }
+  } resolveOnExit{ctx};
 
   if constexpr (std::is_same<T, float>::value) {
     gemm_gpu_widened<float, double>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,

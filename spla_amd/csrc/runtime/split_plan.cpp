@@ -1,0 +1,211 @@
+#include "runtime/split_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace spla_amd {
+
+const int kModuliList[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                     223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+double fp64_bound(int64_t k, int comps) {
+  return static_cast<double>(k) * comps * std::ldexp(1.0, -53);
+}
+
+namespace {
+
+struct RowScan {
+  bool finite = true;
+  bool zero = true;
+  int maxExp = std::numeric_limits<int>::min();
+  double normSq = 0.0;  // ||X||_F^2
+};
+
+RowScan scan(const OperandStats& s) {
+  RowScan r;
+  for (int i = 0; i < s.rows; ++i) {
+    const double mx = s.maxabs[i];
+    if (!std::isfinite(mx) || !std::isfinite(s.sumsq[i])) {
+      r.finite = false;
+      continue;
+    }
+    if (mx > 0.0) {
+      r.zero = false;
+      r.maxExp = std::max(r.maxExp, std::ilogb(mx));
+    }
+    r.normSq += s.sumsq[i];
+  }
+  return r;
+}
+
+// S such that rho(P) = 2^-P * S (see header); computed relative to 2^maxExp.
+double scale_factor(const OperandStats& s, const RowScan& r) {
+  double acc = 0.0;
+  for (int i = 0; i < s.rows; ++i)
+    if (s.maxabs[i] > 0.0) acc += std::ldexp(1.0, 2 * (std::ilogb(s.maxabs[i]) - r.maxExp));
+  const double comps_k = static_cast<double>(s.comps) * static_cast<double>(s.k);
+  // sqrt(comps k sum_i 4^fl_i) / ||X|| with 4^fl_i = 4^maxExp * 4^(fl_i - maxExp)
+  return std::ldexp(std::sqrt(comps_k * acc), r.maxExp) / std::sqrt(r.normSq);
+}
+
+void exponents(const OperandStats& s, int P, std::vector<int>& e) {
+  e.assign(s.rows, 0);
+  for (int i = 0; i < s.rows; ++i)
+    if (s.maxabs[i] > 0.0) e[i] = P - 1 - std::ilogb(s.maxabs[i]);
+}
+
+// log2 of an upper bound of the Euclidean norm of the scaled integer rows.
+double max_row_norm_log2(const OperandStats& s, const std::vector<int>& e, int P) {
+  const double comps_k = static_cast<double>(s.comps) * static_cast<double>(s.k);
+  const double capLog2 = P + 0.5 * std::log2(comps_k);  // every |a| <= 2^P
+  double best = -std::numeric_limits<double>::infinity();
+  for (int i = 0; i < s.rows; ++i) {
+    if (!(s.maxabs[i] > 0.0)) continue;
+    double est = capLog2;
+    if (s.sumsq[i] > 0.0) {
+      // 2^e ||x|| + 0.5 sqrt(comps k), in log2
+      const double a = e[i] + 0.5 * std::log2(s.sumsq[i]);
+      const double b = std::log2(0.5 * std::sqrt(comps_k));
+      const double hi = std::max(a, b), lo = std::min(a, b);
+      est = std::min(capLog2, hi + std::log2(1.0 + std::exp2(lo - hi)));
+    }
+    best = std::max(best, est);
+  }
+  return best;
+}
+
+}  // namespace
+
+SplitPlan plan_split(const OperandStats& A, const OperandStats& B, double tolerance,
+                     int maxModuli) {
+  SplitPlan p;
+  p.tolerance = tolerance;
+  const RowScan ra = scan(A), rb = scan(B);
+  if (!ra.finite || !rb.finite) {
+    p.reason = SplitNonFinite;
+    return p;
+  }
+  if (ra.zero || rb.zero) {
+    p.reason = SplitZeroOperand;
+    return p;
+  }
+  if (!(ra.normSq > 0.0) || !std::isfinite(ra.normSq) || !(rb.normSq > 0.0) ||
+      !std::isfinite(rb.normSq)) {
+    p.reason = SplitRange;
+    return p;
+  }
+  const double SA = scale_factor(A, ra), SB = scale_factor(B, rb);
+  if (!std::isfinite(SA) || !std::isfinite(SB)) {
+    p.reason = SplitRange;
+    return p;
+  }
+  // rho_A + rho_B + rho_A rho_B <= tol with rho_A, rho_B <= tol / 2.02
+  const double half = tolerance / 2.02;
+  p.PA = std::max(1, static_cast<int>(std::ceil(std::log2(SA / half))));
+  p.PB = std::max(1, static_cast<int>(std::ceil(std::log2(SB / half))));
+  p.rhoA = std::ldexp(SA, -p.PA);
+  p.rhoB = std::ldexp(SB, -p.PB);
+  p.bound = p.rhoA + p.rhoB + p.rhoA * p.rhoB;
+  if (p.PA > 53 || p.PB > 53) {
+    p.reason = SplitTooWide;
+    return p;
+  }
+  exponents(A, p.PA, p.expA);
+  exponents(B, p.PB, p.expB);
+  // M > 2 max |a_i . b_j| with margin for the estimate itself
+  p.bitsNeeded = max_row_norm_log2(A, p.expA, p.PA) + max_row_norm_log2(B, p.expB, p.PB) + 2.0;
+  double bits = 0.0;
+  int n = 0;
+  while (n < kMaxModuli && bits < p.bitsNeeded) bits += std::log2(kModuliList[n++]);
+  if (bits < p.bitsNeeded || n > maxModuli) {
+    p.reason = SplitTooManyModuli;
+    p.moduli = n;
+    return p;
+  }
+  p.moduli = n;
+  p.ok = true;
+  p.reason = SplitOk;
+  return p;
+}
+
+void make_moduli(int count, ModuliSet& ms) {
+  std::memset(&ms, 0, sizeof(ms));
+  ms.count = count;
+  for (int t = 0; t < count; ++t) {
+    ms.m[t] = kModuliList[t];
+    ms.inv[t] = 1.0 / kModuliList[t];
+  }
+}
+
+namespace {
+
+struct U192 {
+  uint64_t w[3] = {0, 0, 0};
+};
+
+void mul_small(U192& x, uint32_t s) {
+  unsigned __int128 carry = 0;
+  for (int i = 0; i < 3; ++i) {
+    const unsigned __int128 p = static_cast<unsigned __int128>(x.w[i]) * s + carry;
+    x.w[i] = static_cast<uint64_t>(p);
+    carry = p >> 64;
+  }
+}
+
+uint32_t divmod_small(U192& x, uint32_t s) {
+  unsigned __int128 rem = 0;
+  for (int i = 2; i >= 0; --i) {
+    const unsigned __int128 cur = (rem << 64) | x.w[i];
+    x.w[i] = static_cast<uint64_t>(cur / s);
+    rem = cur % s;
+  }
+  return static_cast<uint32_t>(rem);
+}
+
+uint32_t mod_small(U192 x, uint32_t s) { return divmod_small(x, s); }
+
+int64_t inverse_mod(int64_t a, int64_t m) {
+  int64_t t = 0, newt = 1, r = m, newr = a % m;
+  while (newr != 0) {
+    const int64_t q = r / newr;
+    const int64_t tt = t - q * newt;
+    t = newt;
+    newt = tt;
+    const int64_t rr = r - q * newr;
+    r = newr;
+    newr = rr;
+  }
+  if (t < 0) t += m;
+  return t;
+}
+
+double to_double(const U192& x) {
+  return std::ldexp(static_cast<double>(x.w[2]), 128) + std::ldexp(static_cast<double>(x.w[1]), 64) +
+         static_cast<double>(x.w[0]);
+}
+
+}  // namespace
+
+void make_crt_constants(const ModuliSet& ms, CrtConstants& cc) {
+  std::memset(&cc, 0, sizeof(cc));
+  U192 M;
+  M.w[0] = 1;
+  for (int t = 0; t < ms.count; ++t) mul_small(M, static_cast<uint32_t>(ms.m[t]));
+  for (int t = 0; t < ms.count; ++t) {
+    U192 Mt = M;
+    divmod_small(Mt, static_cast<uint32_t>(ms.m[t]));  // exact division
+    const uint32_t r = mod_small(Mt, static_cast<uint32_t>(ms.m[t]));
+    const int64_t y = inverse_mod(r, ms.m[t]);
+    mul_small(Mt, static_cast<uint32_t>(y));
+    for (int i = 0; i < 3; ++i) cc.W[t][i] = Mt.w[i];
+  }
+  for (int i = 0; i < 3; ++i) cc.M[i] = M.w[i];
+  U192 h = M;
+  divmod_small(h, 2);
+  for (int i = 0; i < 3; ++i) cc.halfM[i] = h.w[i];
+  cc.Md = to_double(M);
+}
+
+}  // namespace spla_amd

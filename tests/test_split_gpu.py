@@ -1,0 +1,218 @@
+"""Split-precision (Ozaki scheme II, INT8 matrix cores) products on the GPU.
+
+* bit-exact against the exact integer restatement oracle/ozaki.py (alpha = 1,
+  beta = 0), for every operand layout, real and complex: this pins the
+  residues, the INT8 GEMM, the CRT reconstruction and the conversion;
+* normwise error ||C - C_dd|| / (||A|| ||B||) against the double-double oracle
+  is below the plan's reported bound and below 1e-13, including the
+  exponent-spread inputs uniform[-1,1] * 2^e, e in [-10, 10];
+* split-K (k > 2^17 - 128 forces INT32 split sums), fill modes, beta != 0,
+  host-resident operands, run-to-run determinism;
+* fall-back to FP64 for non-finite inputs, zero operands, too wide exponent
+  ranges and, in automatic mode, when FP64 is expected to be faster.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import spla_amd as spla
+from helpers import OPS, make_buffer, op_shape, rel_error, to_device, to_matrix
+from oracle import ozaki
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def sctx(device):
+    c = spla.Context(spla.ProcessingUnit.GPU)
+    c.set_gemm_mode(spla.GemmMode.SPLIT_INT8)
+    return c
+
+
+def spread_buffer(seed, mid, rows, cols, ld, dtype, spread=10):
+    rng = np.random.default_rng(seed * 17 + mid)
+    e = rng.integers(-spread, spread + 1, size=ld * cols)
+    v = rng.uniform(-1, 1, size=ld * cols) * np.exp2(e)
+    if np.dtype(dtype).kind == "c":
+        v = v + 1j * rng.uniform(-1, 1, size=ld * cols) * np.exp2(e)
+    return v.astype(dtype)
+
+
+def gpu_gemm(ctx, device, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    dC = to_device(C, device)
+    spla.gemm(opA, opB, m, n, k, alpha, to_device(A, device), lda, to_device(B, device), ldb, beta,
+              dC, ldc, ctx)
+    return dC.cpu().numpy()
+
+
+LAYOUTS = [("C", "N"), ("T", "N"), ("N", "N"), ("C", "C"), ("N", "T"), ("T", "C")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
+@pytest.mark.parametrize("opA,opB", LAYOUTS)
+@pytest.mark.parametrize("shape", [(13, 11, 70), (37, 29, 300), (70, 3, 1)])
+def test_split_bit_exact_against_integer_restatement(sctx, device, dtype, opA, opB, shape):
+    if dtype == np.float64 and "C" in (opA, opB):
+        pytest.skip("conjugation is the identity for real operands")
+    m, n, k = shape
+    oa, ob = OPS[opA], OPS[opB]
+    ra, ca = op_shape(oa, m, k)
+    rb, cb = op_shape(ob, k, n)
+    A = make_buffer(3, 1, ra, ca, ra + 2, dtype)
+    B = make_buffer(3, 2, rb, cb, rb + 1, dtype)
+    C0 = np.zeros(m * n, dtype=dtype)
+    got = gpu_gemm(sctx, device, oa, ob, m, n, k, 1.0, A, ra + 2, B, rb + 1, 0.0, C0, m)
+    info = sctx.last_product_info()
+    assert info["split"], info
+    exp = ozaki.emulate(oa, ob, m, n, k, A, ra + 2, B, rb + 1, info["bits_a"], info["bits_b"])
+    assert np.array_equal(got.reshape(n, m).T, exp)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
+@pytest.mark.parametrize("opA,opB", LAYOUTS)
+@pytest.mark.parametrize("data", ["uniform", "spread"])
+def test_split_accuracy(sctx, device, dtype, opA, opB, data):
+    if dtype == np.float64 and "C" in (opA, opB):
+        pytest.skip("conjugation is the identity for real operands")
+    m, n, k = 150, 130, 4000
+    oa, ob = OPS[opA], OPS[opB]
+    ra, ca = op_shape(oa, m, k)
+    rb, cb = op_shape(ob, k, n)
+    mk = spread_buffer if data == "spread" else make_buffer
+    A = mk(5, 1, ra, ca, ra, dtype)
+    B = mk(5, 2, rb, cb, rb, dtype)
+    C0 = make_buffer(5, 3, m, n, m, dtype)
+    alpha = (0.75 - 0.5j) if dtype == np.complex128 else 0.75
+    beta = (0.5 + 0.25j) if dtype == np.complex128 else 0.5
+    got = gpu_gemm(sctx, device, oa, ob, m, n, k, alpha, A, ra, B, rb, beta, C0, m)
+    info = sctx.last_product_info()
+    assert info["split"], info
+    ref = C0.copy()
+    oracle.gemm(oa, ob, m, n, k, alpha, A, ra, B, rb, beta, ref, m, accurate=True)
+    err = rel_error(got, ref, A, B) / abs(alpha)
+    assert err <= info["bound"] + 1e-16, (err, info)
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
+def test_split_k_beyond_int32_range(sctx, device, dtype):
+    m, n, k = 40, 24, 140000  # > 2^17 - 128: INT32 sums are split
+    A = make_buffer(8, 1, k, m, k, dtype)
+    B = make_buffer(8, 2, k, n, k, dtype)
+    C0 = np.zeros(m * n, dtype=dtype)
+    got = gpu_gemm(sctx, device, 2 if dtype == np.complex128 else 1, 0, m, n, k, 1.0, A, k, B, k,
+                   0.0, C0, m)
+    info = sctx.last_product_info()
+    assert info["split"]
+    ref = C0.copy()
+    oracle.gemm(2 if dtype == np.complex128 else 1, 0, m, n, k, 1.0, A, k, B, k, 0.0, ref, m,
+                accurate=True)
+    assert rel_error(got, ref, A, B) <= min(info["bound"] + 1e-16, 1e-13)
+
+
+def test_split_deterministic(sctx, device):
+    m, n, k = 200, 100, 20000
+    A = to_device(make_buffer(9, 1, k, m, k, np.complex128), device)
+    B = to_device(make_buffer(9, 2, k, n, k, np.complex128), device)
+    outs = []
+    for _ in range(2):
+        dC = to_device(np.zeros(m * n, dtype=np.complex128), device)
+        spla.gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.0, dC, m, sctx)
+        outs.append(dC.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("fill", [spla.FillMode.UPPER, spla.FillMode.LOWER])
+def test_split_ssbtr_fill(sctx, device, fill):
+    m, n, k, ro, co = 600, 520, 900, 7, 300
+    ldc = m + ro + 5
+    A = make_buffer(12, 1, k, m, k, np.complex128)
+    B = make_buffer(12, 2, k, n, k, np.complex128)
+    C0 = make_buffer(12, 3, ldc, n + co, ldc, np.complex128)
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 64, 64)
+    dC = to_device(C0, device)
+    spla.pgemm_ssbtr(m, n, k, 2, 1.0, to_device(A, device), k, to_device(B, device), k, 0.0, dC,
+                     ldc, ro, co, fill, dist, sctx)
+    assert sctx.last_product_info()["split"]
+    got = to_matrix(dC.cpu().numpy(), ldc, n + co, ldc)
+    ref_full = C0.copy()
+    oracle.gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.0, ref_full[ro + co * ldc:], ldc, accurate=True)
+    R = to_matrix(ref_full, ldc, n + co, ldc)
+    rows = np.arange(ldc)[:, None]
+    cols = np.arange(n + co)[None, :]
+    inside = (rows >= ro) & (rows < ro + m) & (cols >= co)
+    req = inside & ((cols >= rows) if fill == spla.FillMode.UPPER else (rows >= cols))
+    assert rel_error(got[req], R[req], A, B) <= 1e-13
+    O = to_matrix(C0, ldc, n + co, ldc)
+    assert np.array_equal(got[~inside], O[~inside])
+
+
+def test_split_host_resident_operands(sctx, device):
+    m, n, k = 128, 96, 6000
+    A = make_buffer(13, 1, k, m, k, np.complex128)
+    B = make_buffer(13, 2, k, n, k, np.complex128)
+    C = make_buffer(13, 3, m, n, m, np.complex128)
+    ref = C.copy()
+    sctx.set_stream_chunk_bytes(1 << 20)  # several k-chunks
+    dist = spla.MatrixDistribution.create_mirror(spla.MPI_COMM_SELF)
+    spla.pgemm_ssb(m, n, k, 2, 1.0, A, k, B, k, 0.5, C, m, 0, 0, dist, sctx)
+    oracle.gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.5, ref, m, accurate=True)
+    assert rel_error(C, ref, A, B) <= 1e-13
+
+
+def test_split_fallbacks(sctx, device):
+    m, n, k = 20, 16, 50000
+    A = make_buffer(14, 1, k, m, k, np.float64)
+    B = make_buffer(14, 2, k, n, k, np.float64)
+    C0 = np.zeros(m * n)
+    # non-finite input: FP64 kernels (NaN propagates as in FP64)
+    An = A.copy()
+    An[5] = np.nan
+    got = gpu_gemm(sctx, device, 1, 0, m, n, k, 1.0, An, k, B, k, 0.0, C0, m)
+    info = sctx.last_product_info()
+    assert not info["split"] and info["reason"] == 1
+    G = got.reshape(n, m)
+    assert np.isnan(G[:, 0]).all() and np.isfinite(G[:, 1:]).all()
+    # zero operand
+    gpu_gemm(sctx, device, 1, 0, m, n, k, 1.0, np.zeros_like(A), k, B, k, 0.0, C0, m)
+    info = sctx.last_product_info()
+    assert not info["split"] and info["reason"] == 2
+    # one dominant entry per row: the normwise bound needs more than 53 bits
+    Aw = np.full(k * m, 1e-40)
+    Aw[::k] = 1.0
+    got = gpu_gemm(sctx, device, 1, 0, m, n, k, 1.0, Aw, k, B, k, 0.0, C0, m)
+    info = sctx.last_product_info()
+    assert not info["split"] and info["reason"] == 3
+    ref = C0.copy()
+    oracle.gemm(1, 0, m, n, k, 1.0, Aw, k, B, k, 0.0, ref, m, accurate=True)
+    assert rel_error(got, ref, Aw, B) <= 1e-13
+
+
+def test_auto_mode_keeps_small_products_on_fp64(device):
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_gemm_mode(spla.GemmMode.AUTO)
+    m, n, k = 32, 32, 64
+    A = make_buffer(15, 1, k, m, k, np.complex128)
+    B = make_buffer(15, 2, k, n, k, np.complex128)
+    gpu_gemm(ctx, device, 2, 0, m, n, k, 1.0, A, k, B, k, 0.0, np.zeros(m * n, complex), m)
+    info = ctx.last_product_info()
+    assert not info["split"] and info["reason"] == 6
+    ctx.set_gemm_mode(spla.GemmMode.FP64)
+    gpu_gemm(ctx, device, 2, 0, m, n, k, 1.0, A, k, B, k, 0.0, np.zeros(m * n, complex), m)
+    assert ctx.last_product_info()["reason"] == 7
+
+
+def test_phase_timing_reports_split_phases(sctx, device):
+    sctx.set_timing(True)
+    m, n, k = 256, 256, 8192
+    A = to_device(make_buffer(16, 1, k, m, k, np.complex128), device)
+    B = to_device(make_buffer(16, 2, k, n, k, np.complex128), device)
+    dC = to_device(np.zeros(m * n, dtype=np.complex128), device)
+    spla.gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.0, dC, m, sctx)
+    t = sctx.last_timing()
+    assert t["split_int8_gemm"] > 0 and t["split_residues"] > 0 and t["split_crt"] > 0
+    assert t["gemm_fp64"] == 0
+    sctx.set_timing(False)
