@@ -67,6 +67,65 @@ SPLA_EXPORT SplaError spla_amd_ctx_last_product_info(SplaContext ctx, int* used,
                                                      int* moduli, int* bitsA, int* bitsB,
                                                      double* bound);
 
+/* ------------------------------------------------------------------------
+ * Batched ring-step block products.
+ *
+ * One ring step of pgemm_ssb / pgemm_ssbtr forms many independent block
+ * products C_b <- alpha op(A_b) B_b + beta C_b with op(A) = A^T or A^H
+ * (SPLA src/pgemm_ssb/ring_ssb_gpu.cpp:193-197 issues one vendor GEMM per
+ * block). A batch runs all of them in one persistent launch captured in a
+ * hipGraph; tile shape and per-block split-K factors come from the library's
+ * autotuner. All pointers must be device memory. A block's fill mode and
+ * offsets follow pgemm_ssbtr: tiles wholly outside the requested triangle of
+ * the global C are skipped.
+ */
+typedef void* SplaAmdBatch;
+
+typedef struct SplaAmdBlock {
+  int m, n, k;        /* C block m x n, reduction length k */
+  const void* A;      /* k x m, leading dimension lda >= k */
+  int lda;
+  const void* B;      /* k x n, leading dimension ldb >= k */
+  int ldb;
+  void* C;            /* m x n, leading dimension ldc >= m */
+  int ldc;
+  SplaFillMode fillMode;
+  int rowOffset;      /* position of the block in the global C */
+  int colOffset;
+} SplaAmdBlock;
+
+enum SplaAmdElement { SPLA_AMD_DOUBLE = 0, SPLA_AMD_COMPLEX_DOUBLE = 1 };
+
+enum SplaAmdTuning {
+  SPLA_AMD_TUNE_MODEL = 0,   /* cost model of the persistent schedule */
+  SPLA_AMD_TUNE_MEASURE = 1  /* also time candidate plans on scratch copies of C */
+};
+
+#ifndef __cplusplus
+typedef struct SplaAmdBlock SplaAmdBlock;
+typedef enum SplaAmdElement SplaAmdElement;
+typedef enum SplaAmdTuning SplaAmdTuning;
+#endif
+
+/* Plans (and autotunes) a batch; alpha / beta point to one double or one
+ * (re, im) pair. The block array is copied. */
+SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext ctx,
+                                                SplaAmdElement type, SplaOperation opA, int count,
+                                                const SplaAmdBlock* blocks, const void* alpha,
+                                                const void* beta, SplaAmdTuning tuning);
+
+/* Runs the batch (synchronous). In split-precision mode every block runs as
+ * its own split product instead of the captured FP64 launch. */
+SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch);
+
+/* Plan summary: tile configuration (0: 128 x 128, 1: 64 x 64 embedded), work
+ * items, workgroups of the persistent launch, modelled and measured (0 when
+ * not measured) milliseconds. Output pointers may be NULL. */
+SPLA_EXPORT SplaError spla_amd_batch_info(SplaAmdBatch batch, int* tileConfig, int* items,
+                                          int* workgroups, double* modelMs, double* measuredMs);
+
+SPLA_EXPORT SplaError spla_amd_batch_destroy(SplaAmdBatch* batch);
+
 /* Kernel phases reported by spla_amd_ctx_last_timing. */
 enum SplaAmdPhase {
   SPLA_AMD_PHASE_GEMM_FP64 = 0,        /* FP64 MFMA GEMM kernels */

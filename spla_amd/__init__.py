@@ -7,13 +7,18 @@ and C++ API over hand-written HIP kernels); this package is its Python mirror.
 from spla_amd.api import (  # noqa: F401
     MPI_COMM_SELF,
     MPI_COMM_WORLD,
+    Batch,
     Context,
     DistributionType,
     FillMode,
     GemmMode,
     GenericError,
     GPUAllocationError,
+    GPUBlasError,
     GPUError,
+    GPUInvalidDevicePointerError,
+    GPUInvalidValueError,
+    GPULaunchError,
     GPUNoDeviceError,
     GPUSupportError,
     InternalError,
@@ -22,7 +27,9 @@ from spla_amd.api import (  # noqa: F401
     InvalidParameterError,
     InvalidPointerError,
     MatrixDistribution,
+    MPIAllocError,
     MPIError,
+    MPIThreadSupportError,
     Operation,
     ProcessingUnit,
     SplaError,

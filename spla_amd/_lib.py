@@ -26,6 +26,15 @@ MPI_Comm = ctypes.c_int
 MPI_COMM_WORLD = 0x44000000
 MPI_COMM_SELF = 0x44000001
 
+
+
+class SplaAmdBlock(ctypes.Structure):
+    """struct SplaAmdBlock of include/spla_amd/spla_amd.h."""
+    _fields_ = [("m", c_int), ("n", c_int), ("k", c_int), ("A", c_void_p), ("lda", c_int),
+                ("B", c_void_p), ("ldb", c_int), ("C", c_void_p), ("ldc", c_int),
+                ("fillMode", c_int), ("rowOffset", c_int), ("colOffset", c_int)]
+
+
 _lib = None
 
 
@@ -86,6 +95,11 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("spla_amd_ctx_set_split_tolerance", c_void_p, c_double)
     fn("spla_amd_ctx_set_timing", c_void_p, c_int)
     fn("spla_amd_ctx_last_timing", c_void_p, P(c_double))
+    fn("spla_amd_batch_create_ssb", P(c_void_p), c_void_p, c_int, c_int, c_int, c_void_p,
+       c_void_p, c_void_p, c_int)
+    fn("spla_amd_batch_execute", c_void_p)
+    fn("spla_amd_batch_info", c_void_p, P(c_int), P(c_int), P(c_int), P(c_double), P(c_double))
+    fn("spla_amd_batch_destroy", P(c_void_p))
     fn("spla_amd_ctx_last_product_info", c_void_p, P(c_int), P(c_int), P(c_int), P(c_int),
        P(c_int), P(c_double))
 

@@ -318,6 +318,55 @@ class MatrixDistribution:
         check(self._lib.spla_mat_dis_set_col_block_size(self._h, v))
 
 
+class Batch:
+    """Batched ring-step block products (spla_amd_batch_* of include/spla_amd/spla_amd.h).
+
+    blocks: sequence of dicts with keys m, n, k, A, lda, B, ldb, C, ldc and optional
+    fill (FillMode), row_offset, col_offset; arrays are device tensors or raw
+    device addresses.
+    """
+
+    def __init__(self, ctx: Context, blocks, alpha, beta, complex_: bool = True,
+                 opA: Operation = Operation.CONJ_TRANSPOSE, measure: bool = False):
+        self._lib = ctx._lib
+        self._ctx = ctx  # keeps the context alive
+        arr = (_lib.SplaAmdBlock * max(1, len(blocks)))()
+        for i, b in enumerate(blocks):
+            arr[i] = _lib.SplaAmdBlock(b["m"], b["n"], b["k"], _ptr(b["A"]), b["lda"],
+                                       _ptr(b["B"]), b["ldb"], _ptr(b["C"]), b["ldc"],
+                                       int(b.get("fill", FillMode.FULL)),
+                                       b.get("row_offset", 0), b.get("col_offset", 0))
+        dt = np.complex128 if complex_ else np.float64
+        self._alpha = np.array([alpha], dtype=dt)
+        self._beta = np.array([beta], dtype=dt)
+        self._h = ctypes.c_void_p()
+        check(self._lib.spla_amd_batch_create_ssb(
+            ctypes.byref(self._h), ctx.handle, 1 if complex_ else 0, int(opA), len(blocks), arr,
+            self._alpha.ctypes.data, self._beta.ctypes.data, 1 if measure else 0))
+
+    def execute(self) -> None:
+        check(self._lib.spla_amd_batch_execute(self._h))
+
+    def info(self) -> dict:
+        cfg, items, wgs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        model, meas = ctypes.c_double(), ctypes.c_double()
+        check(self._lib.spla_amd_batch_info(self._h, ctypes.byref(cfg), ctypes.byref(items),
+                                            ctypes.byref(wgs), ctypes.byref(model),
+                                            ctypes.byref(meas)))
+        return {"tile_config": cfg.value, "items": items.value, "workgroups": wgs.value,
+                "model_ms": model.value, "measured_ms": meas.value}
+
+    def close(self) -> None:
+        if self._h:
+            check(self._lib.spla_amd_batch_destroy(ctypes.byref(self._h)))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+
 # ---------------------------------------------------------------------------
 # array plumbing
 

@@ -1,5 +1,6 @@
 // Entry points of include/spla_amd/spla_amd.h (no SPLA counterpart).
 #include "api/internal_access.hpp"
+#include "runtime/batched.hpp"
 #include "spla_amd/spla_amd.h"
 
 extern "C" {
@@ -60,6 +61,55 @@ SPLA_EXPORT SplaError spla_amd_ctx_last_timing(SplaContext ctx, double* ms) {
   const auto& t = spla::InternalAccess::ctx(*reinterpret_cast<spla::Context*>(ctx)).timer().ms();
   for (int i = 0; i < SPLA_AMD_NUM_PHASES; ++i) ms[i] = t[i];
   return SPLA_SUCCESS;
+}
+
+SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext ctx,
+                                                SplaAmdElement type, SplaOperation opA, int count,
+                                                const SplaAmdBlock* blocks, const void* alpha,
+                                                const void* beta, SplaAmdTuning tuning) {
+  if (!batch || !ctx) return SPLA_INVALID_HANDLE_ERROR;
+  if (count < 0 || (type != SPLA_AMD_DOUBLE && type != SPLA_AMD_COMPLEX_DOUBLE))
+    return SPLA_INVALID_PARAMETER_ERROR;
+  if ((count > 0 && !blocks) || !alpha || !beta) return SPLA_INVALID_POINTER_ERROR;
+  SPLA_C_GUARD({
+    std::vector<spla_amd::BlockSpec> specs(count);
+    for (int i = 0; i < count; ++i) {
+      const SplaAmdBlock& b = blocks[i];
+      specs[i] = spla_amd::BlockSpec{b.m,   b.n,   b.k,   b.A,          b.lda,        b.B,
+                                     b.ldb, b.C,   b.ldc, static_cast<int>(b.fillMode),
+                                     b.rowOffset, b.colOffset};
+    }
+    auto ctxPtr = spla::InternalAccess::ctx_ptr(*reinterpret_cast<spla::Context*>(ctx));
+    *batch = new spla_amd::BatchPlan(ctxPtr, type == SPLA_AMD_COMPLEX_DOUBLE, opA,
+                                     std::move(specs), static_cast<const double*>(alpha),
+                                     static_cast<const double*>(beta),
+                                     tuning == SPLA_AMD_TUNE_MEASURE);
+  })
+}
+
+SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch) {
+  if (!batch) return SPLA_INVALID_HANDLE_ERROR;
+  SPLA_C_GUARD(static_cast<spla_amd::BatchPlan*>(batch)->execute())
+}
+
+SPLA_EXPORT SplaError spla_amd_batch_info(SplaAmdBatch batch, int* tileConfig, int* items,
+                                          int* workgroups, double* modelMs, double* measuredMs) {
+  if (!batch) return SPLA_INVALID_HANDLE_ERROR;
+  const auto* p = static_cast<const spla_amd::BatchPlan*>(batch);
+  if (tileConfig) *tileConfig = p->tile_config();
+  if (items) *items = p->num_items();
+  if (workgroups) *workgroups = p->workgroups();
+  if (modelMs) *modelMs = p->model_ms();
+  if (measuredMs) *measuredMs = p->measured_ms();
+  return SPLA_SUCCESS;
+}
+
+SPLA_EXPORT SplaError spla_amd_batch_destroy(SplaAmdBatch* batch) {
+  if (!batch) return SPLA_INVALID_HANDLE_ERROR;
+  SPLA_C_GUARD({
+    delete static_cast<spla_amd::BatchPlan*>(*batch);
+    *batch = nullptr;
+  })
 }
 
 }  // extern "C"

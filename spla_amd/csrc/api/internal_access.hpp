@@ -13,6 +13,7 @@ namespace spla {
 
 struct InternalAccess {
   static ContextInternal& ctx(Context& c) { return *c.ctxInternal_; }
+  static std::shared_ptr<ContextInternal> ctx_ptr(Context& c) { return c.ctxInternal_; }
   static MatrixDistributionInternal& dist(MatrixDistribution& d) { return *d.descInternal_; }
   static MatrixDistribution make_dist(MatrixDistributionInternal&& d) {
     return MatrixDistribution(std::make_shared<MatrixDistributionInternal>(std::move(d)));

@@ -33,81 +33,12 @@
 
 #include "kernels/gemm_desc.hpp"
 #include "kernels/launch.hpp"
+#include "kernels/tile_mma.hpp"
 
 namespace spla_amd {
 namespace {
 
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef double dbl2 __attribute__((ext_vector_type(2)));
-
-constexpr int kThreads = 256;
-constexpr int kPad = 2;  // LDS row padding (doubles): keeps 16-byte alignment, spreads banks
-
-__device__ __forceinline__ double flip_sign(double x, uint32_t hiMask) {
-  uint64_t b = __builtin_bit_cast(uint64_t, x);
-  b ^= static_cast<uint64_t>(hiMask) << 32;
-  return __builtin_bit_cast(double, b);
-}
-
-// Stages one operand tile (ROWS stored rows x BK embedded reduction elements)
-// from global memory into registers and from registers into LDS.
-template <bool CPLX, int LAYOUT, int ROWS, int BK>
-struct TileStage {
-  static constexpr int kPairs = ROWS * BK / 2;
-  static constexpr int kPerThread = kPairs / kThreads;
-  static_assert(kPairs % kThreads == 0, "tile does not divide over the workgroup");
-  dbl2 regs[kPerThread];
-
-  __device__ __forceinline__ static void coords(int pi, int& row, int& qp) {
-    if (LAYOUT == KMajor) {
-      row = pi / (BK / 2);
-      qp = pi % (BK / 2);
-    } else {
-      row = pi % ROWS;
-      qp = pi / ROWS;
-    }
-  }
-
-  // base: operand origin (doubles); ld: leading dimension in elements;
-  // rowBase/rowLimit: stored-row range; q0/qEnd: embedded reduction range.
-  __device__ __forceinline__ void load(const double* __restrict__ base, int64_t ld, int rowBase,
-                                       int rowLimit, int64_t q0, int64_t qEnd, int tid) {
-#pragma unroll
-    for (int u = 0; u < kPerThread; ++u) {
-      int row, qp;
-      coords(tid + u * kThreads, row, qp);
-      const int grow = rowBase + row;
-      const int64_t q = q0 + 2 * qp;
-      double d0 = 0.0, d1 = 0.0;
-      if (grow < rowLimit) {
-        if (LAYOUT == KMajor) {
-          const double* p = base + static_cast<int64_t>(grow) * ld * (CPLX ? 2 : 1) + q;
-          if (q < qEnd) d0 = p[0];
-          if (q + 1 < qEnd) d1 = p[1];
-        } else if (CPLX) {
-          if (q < qEnd) {
-            const double* p = base + ((q >> 1) * ld + grow) * 2;
-            d0 = p[0];
-            d1 = p[1];
-          }
-        } else {
-          if (q < qEnd) d0 = base[q * ld + grow];
-          if (q + 1 < qEnd) d1 = base[(q + 1) * ld + grow];
-        }
-      }
-      regs[u] = dbl2{d0, d1};
-    }
-  }
-
-  __device__ __forceinline__ void store(double (*lds)[BK + kPad], int tid) const {
-#pragma unroll
-    for (int u = 0; u < kPerThread; ++u) {
-      int row, qp;
-      coords(tid + u * kThreads, row, qp);
-      *reinterpret_cast<dbl2*>(&lds[row][2 * qp]) = regs[u];
-    }
-  }
-};
+using namespace tile;
 
 __device__ __forceinline__ bool tile_active(const GemmDesc& d, int64_t r, int64_t c, int64_t nr,
                                             int64_t nc) {
@@ -159,166 +90,31 @@ __device__ __forceinline__ void map_block(const GemmDesc& d, int& s, int& tm, in
 
 template <bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK, int MINBLOCKS>
 __global__ void __launch_bounds__(kThreads, MINBLOCKS) gemm_f64_kernel(GemmDesc d) {
-  constexpr int WAVES_N = 4 / WAVES_M;
-  constexpr int BM = WM * WAVES_M;
-  constexpr int BN = WN * WAVES_N;
-  constexpr int AR = CPLX ? BM / 2 : BM;
-  constexpr int FM = WM / 16;
-  constexpr int FN = WN / 16;
-  static_assert(BK % 8 == 0, "k-block must hold whole 8-element sub-blocks");
-
-  __shared__ double lA[2][AR][BK + kPad];
-  __shared__ double lB[2][BN][BK + kPad];
+  using T = TileMma<CPLX, LA, LB, WM, WN, WAVES_M, BK>;
+  __shared__ typename T::Smem sm;
 
   int s, tm, tn;
   map_block(d, s, tm, tn);
-  const int row0 = tm * BM;  // embedded rows
-  const int col0 = tn * BN;
+  const int row0 = tm * T::BM;  // embedded rows
+  const int col0 = tn * T::BN;
   const int arow0 = CPLX ? row0 / 2 : row0;
-
   if (d.fillMode != FillFull) {
-    const int64_t nr = min(AR, d.m - arow0);
-    const int64_t nc = min(BN, d.n - col0);
+    const int64_t nr = min(T::AR, d.m - arow0);
+    const int64_t nc = min(T::BN, d.n - col0);
     if (!tile_active(d, arow0, col0, nr, nc)) return;
   }
-
   const int64_t kBegin = static_cast<int64_t>(s) * d.kSplit;
   const int64_t kEnd = min(d.Kp, kBegin + d.kSplit);
-  const int nk = static_cast<int>((kEnd - kBegin + BK - 1) / BK);
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wid = tid >> 6;
-  const int wm = wid % WAVES_M;
-  const int wn = wid / WAVES_M;
-  const int l16 = lane & 15;
-  const int kk = lane >> 4;
-
-  // Lane-constant complex operand selection (see gemm_desc.hpp):
-  //   real row  (t = 0): step0 <- Re(a),  step1 <- Im(a) * (conjA == conjB ? -1 : 1)
-  //   imag row  (t = 1): step0 <- Im(a) * (conjA ? -1 : 1), step1 <- Re(a) * (conjB ? -1 : 1)
-  // Fragment row r = lane % 16 is an imaginary row for r >= 8.
-  bool swapPair = false;
-  uint32_t sgn0 = 0, sgn1 = 0;
-  if (CPLX) {
-    swapPair = l16 >= 8;
-    if (!swapPair) {
-      sgn1 = (d.conjA == d.conjB) ? 0x80000000u : 0u;
-    } else {
-      sgn0 = d.conjA ? 0x80000000u : 0u;
-      sgn1 = d.conjB ? 0x80000000u : 0u;
-    }
-  }
-
-  dbl4 acc[FM][FN];
-#pragma unroll
-  for (int f = 0; f < FM; ++f)
-#pragma unroll
-    for (int g = 0; g < FN; ++g) acc[f][g] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-  TileStage<CPLX, LA, AR, BK> stA;
-  TileStage<CPLX, LB, BN, BK> stB;
-
-  if (nk > 0) {
-    stA.load(d.A, d.lda, arow0, d.m, kBegin, kEnd, tid);
-    stB.load(d.B, d.ldb, col0, d.n, kBegin, kEnd, tid);
-    stA.store(lA[0], tid);
-    stB.store(lB[0], tid);
-  }
-  __syncthreads();
-
-  for (int it = 0; it < nk; ++it) {
-    const int cur = it & 1;
-    const bool more = it + 1 < nk;
-    if (more) {
-      const int64_t q0 = kBegin + static_cast<int64_t>(it + 1) * BK;
-      stA.load(d.A, d.lda, arow0, d.m, q0, kEnd, tid);
-      stB.load(d.B, d.ldb, col0, d.n, q0, kEnd, tid);
-    }
-
-#pragma unroll
-    for (int kb8 = 0; kb8 < BK / 8; ++kb8) {
-      double a0[FM], a1[FM];
-      dbl2 b[FN];
-#pragma unroll
-      for (int f = 0; f < FM; ++f) {
-        const int sr = CPLX ? (wm * WM) / 2 + 8 * f + (l16 & 7) : wm * WM + 16 * f + l16;
-        const dbl2 v = *reinterpret_cast<const dbl2*>(&lA[cur][sr][kb8 * 8 + 2 * kk]);
-        if (CPLX) {
-          const double x = swapPair ? v.y : v.x;
-          const double y = swapPair ? v.x : v.y;
-          a0[f] = flip_sign(x, sgn0);
-          a1[f] = flip_sign(y, sgn1);
-        } else {
-          a0[f] = v.x;
-          a1[f] = v.y;
-        }
-      }
-#pragma unroll
-      for (int g = 0; g < FN; ++g) {
-        const int j = wn * WN + 16 * g + l16;
-        b[g] = *reinterpret_cast<const dbl2*>(&lB[cur][j][kb8 * 8 + 2 * kk]);
-      }
-#pragma unroll
-      for (int f = 0; f < FM; ++f)
-#pragma unroll
-        for (int g = 0; g < FN; ++g)
-          acc[f][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[f], b[g].x, acc[f][g], 0, 0, 0);
-#pragma unroll
-      for (int f = 0; f < FM; ++f)
-#pragma unroll
-        for (int g = 0; g < FN; ++g)
-          acc[f][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[f], b[g].y, acc[f][g], 0, 0, 0);
-    }
-
-    if (more) {
-      stA.store(lA[cur ^ 1], tid);
-      stB.store(lB[cur ^ 1], tid);
-    }
-    __syncthreads();
-  }
-
-  // Accumulator layout of v_mfma_f64_16x16x4_f64: lane holds rows kk + 4 * r,
-  // r = 0..3, of column lane % 16 (kk = lane / 16). For complex fragments this
-  // is (re, im) of complex rows kk and kk + 4 in (acc[0], acc[2]) and
-  // (acc[1], acc[3]).
-#pragma unroll
-  for (int f = 0; f < FM; ++f) {
-#pragma unroll
-    for (int g = 0; g < FN; ++g) {
-      const int j = col0 + wn * WN + 16 * g + l16;
-      if (j >= d.n) continue;
-      const dbl4 v = acc[f][g];
-      if (CPLX) {
-        const int i0 = arow0 + (wm * WM) / 2 + 8 * f + kk;
-        if (d.splits > 1) {
-          double* w = d.work + (static_cast<int64_t>(s) * d.n + j) * d.Mp;
-          if (i0 < d.m) {
-            w[2 * i0] = v[0];
-            w[2 * i0 + 1] = v[2];
-          }
-          if (i0 + 4 < d.m) {
-            w[2 * i0 + 8] = v[1];
-            w[2 * i0 + 9] = v[3];
-          }
-        } else {
-          if (i0 < d.m) store_result<true>(d, i0, j, v[0], v[2]);
-          if (i0 + 4 < d.m) store_result<true>(d, i0 + 4, j, v[1], v[3]);
-        }
-      } else {
-        const int p0 = row0 + wm * WM + 16 * f + kk;
-        if (d.splits > 1) {
-          double* w = d.work + (static_cast<int64_t>(s) * d.n + j) * d.Mp;
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (p0 + 4 * r < d.m) w[p0 + 4 * r] = v[r];
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (p0 + 4 * r < d.m) store_result<false>(d, p0 + 4 * r, j, v[r], 0.0);
-        }
-      }
-    }
+  const TileOperands op{d.A, d.B, d.lda, d.ldb, d.m, d.n, d.conjA, d.conjB};
+  dbl4 acc[T::FM][T::FN];
+  T::run(op, row0, col0, kBegin, kEnd, sm, acc);
+  if (d.splits > 1) {
+    double* w = d.work + static_cast<int64_t>(s) * d.n * d.Mp;
+    T::for_each(row0, col0, d.m, d.n, acc,
+                [&](int i, int j, double re, double im) { store_work<CPLX>(w, d.Mp, i, j, re, im); });
+  } else {
+    T::for_each(row0, col0, d.m, d.n, acc,
+                [&](int i, int j, double re, double im) { store_result<CPLX>(d, i, j, re, im); });
   }
 }
 

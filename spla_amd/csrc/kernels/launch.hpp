@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "kernels/batched_desc.hpp"
 #include "kernels/gemm_desc.hpp"
 #include "kernels/ozaki_desc.hpp"
 
@@ -37,5 +38,9 @@ hipError_t launch_split_stats(bool complex, int layout, const double* X, int64_t
 hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipStream_t stream);
 hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream);
 hipError_t launch_crt(bool complex, const CrtDesc& d, hipStream_t stream);
+
+// Persistent batched block products (batched.hip); cfg is TileL or TileM.
+hipError_t launch_batched_ssb(const BatchDesc& d, bool complex, int cfg, int workgroups,
+                              hipStream_t stream);
 
 }  // namespace spla_amd

@@ -183,6 +183,30 @@ def test_parameter_errors_precede_device_work():
     spla.gemm(0, 0, 0, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2, ctx, dtype="float64")
 
 
+def test_batch_parameter_errors_precede_device_work():
+    ctx = spla.Context(spla.ProcessingUnit.HOST)
+    good = dict(m=4, n=4, k=4, A=1, lda=4, B=2, ldb=4, C=3, ldc=4)
+    with pytest.raises(spla.InvalidParameterError):  # opA must be T or C
+        spla.Batch(ctx, [good], 1.0, 0.0, opA=spla.Operation.NONE)
+    for bad in (dict(m=-1), dict(lda=3), dict(ldb=2), dict(ldc=3), dict(fill=7),
+                dict(row_offset=-1)):
+        with pytest.raises(spla.InvalidParameterError):
+            spla.Batch(ctx, [{**good, **bad}], 1.0, 0.0)
+    with pytest.raises(spla.InvalidPointerError):
+        spla.Batch(ctx, [{**good, "A": None}], 1.0, 0.0)
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    one = (ctypes.c_double * 2)(1.0, 0.0)
+    assert lib.spla_amd_batch_create_ssb(ctypes.byref(h), ctx.handle, 5, 2, 0, None, one, one,
+                                         0) == spla.SplaError.INVALID_PARAMETER_ERROR
+    assert lib.spla_amd_batch_create_ssb(ctypes.byref(h), None, 1, 2, 0, None, one, one,
+                                         0) == spla.SplaError.INVALID_HANDLE_ERROR
+    assert lib.spla_amd_batch_create_ssb(ctypes.byref(h), ctx.handle, 1, 2, 1, None, one, one,
+                                         0) == spla.SplaError.INVALID_POINTER_ERROR
+    assert lib.spla_amd_batch_execute(None) == spla.SplaError.INVALID_HANDLE_ERROR
+    assert lib.spla_amd_batch_destroy(None) == spla.SplaError.INVALID_HANDLE_ERROR
+
+
 MPI_LIB = "/opt/conda/lib/libmpi.so"
 MPIEXEC = "/opt/conda/bin/mpiexec"
 

@@ -1,0 +1,226 @@
+"""Batched ring-step block products (spla_amd_batch_*) on the GPU.
+
+The batch runs many independent C_b <- alpha op(A_b) B_b + beta C_b, the block
+products of one pgemm_ssb / pgemm_ssbtr ring step
+(src/pgemm_ssb/ring_ssb_gpu.cpp:193-197), in one persistent launch replayed
+from a hipGraph. Checked here:
+
+* every block against the double-double oracle (normwise error <= 1e-13),
+  real and complex, opA transpose and conjugate transpose, ragged sizes
+  including m, n or k equal to zero, ld padding and beta != 0;
+* fill modes with tile granularity (SPLA's block_is_active,
+  src/block_generation/block.hpp:58-66): the requested triangle is exact, no
+  element outside the block is written, and elements of the block outside the
+  triangle hold either their old value or the product;
+* split-K items (large k) reduced by the last arriving workgroup: results are
+  bitwise identical over repeated graph replays, with unrelated GPU work in
+  between, and across measured / modelled tuning;
+* the split-precision path of a batch (gemm mode SPLIT_INT8).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import spla_amd as spla
+from helpers import make_buffer, rel_error, to_device, to_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def make_blocks(seed, sizes, dtype, device, fills=None, pad=(0, 0, 0)):
+    """Device operands for blocks (m, n, k); returns (specs, host copies)."""
+    import torch
+
+    specs, host = [], []
+    for i, (m, n, k) in enumerate(sizes):
+        lda, ldb, ldc = max(1, k) + pad[0], max(1, k) + pad[1], max(1, m) + pad[2]
+        A = make_buffer(seed, 3 * i + 1, k, m, lda, dtype)
+        B = make_buffer(seed, 3 * i + 2, k, n, ldb, dtype)
+        C = make_buffer(seed, 3 * i + 3, m, n, ldc, dtype)
+        dA, dB, dC = (to_device(x, device) for x in (A, B, C))
+        fill, ro, co = fills[i] if fills else (spla.FillMode.FULL, 0, 0)
+        specs.append(dict(m=m, n=n, k=k, A=dA, lda=lda, B=dB, ldb=ldb, C=dC, ldc=ldc, fill=fill,
+                          row_offset=ro, col_offset=co))
+        host.append((A, B, C))
+    torch.cuda.synchronize()
+    return specs, host
+
+
+def expected(spec, host, opA, alpha, beta):
+    A, B, C = host
+    ref = C.copy()
+    m, n, k = spec["m"], spec["n"], spec["k"]
+    if m and n:
+        oracle.gemm(opA, 0, m, n, k, alpha, A, spec["lda"], B, spec["ldb"], beta, ref, spec["ldc"],
+                    accurate=True)
+    return ref
+
+
+def check_block(spec, host, opA, alpha, beta, tol=1e-13):
+    A, B, C = host
+    got = spec["C"].cpu().numpy()
+    ref = expected(spec, host, opA, alpha, beta)
+    m, n, ldc = spec["m"], spec["n"], spec["ldc"]
+    if m == 0 or n == 0:
+        assert np.array_equal(got, C)
+        return
+    # ld padding is never written
+    G = got[: ldc * n].reshape(n, ldc).T
+    O = C[: ldc * n].reshape(n, ldc).T
+    assert np.array_equal(G[m:], O[m:])
+    g, r, o = to_matrix(got, m, n, ldc), to_matrix(ref, m, n, ldc), to_matrix(C, m, n, ldc)
+    rows = np.arange(m)[:, None] + spec["row_offset"]
+    cols = np.arange(n)[None, :] + spec["col_offset"]
+    if spec["fill"] == spla.FillMode.UPPER:
+        req = cols >= rows
+    elif spec["fill"] == spla.FillMode.LOWER:
+        req = rows >= cols
+    else:
+        req = np.ones((m, n), dtype=bool)
+    scale = max(abs(alpha), 1e-300)
+    assert rel_error(g[req], r[req], A, B) / scale <= tol
+    # outside the triangle: the old value or the product (whole tiles are computed)
+    rest = ~req
+    if rest.any():
+        old = g[rest] == o[rest]
+        close = np.abs(g[rest] - r[rest]) <= 1e-12 * (np.abs(r[rest]) + 1.0)
+        assert (old | close).all()
+
+
+def ragged_sizes(rng, count, max_m, max_n, max_k):
+    out = []
+    for _ in range(count):
+        out.append((int(rng.integers(1, max_m)), int(rng.integers(1, max_n)),
+                    int(rng.integers(0, max_k))))
+    return out
+
+
+@pytest.mark.parametrize("dtype,opA", [(np.complex128, 2), (np.complex128, 1), (np.float64, 1)],
+                         ids=["zC", "zT", "dT"])
+def test_batch_ragged_blocks(ctx, device, dtype, opA):
+    rng = np.random.default_rng(41)
+    sizes = ragged_sizes(rng, 40, 300, 220, 2500)
+    sizes += [(0, 17, 30), (19, 0, 30), (23, 31, 0), (1, 1, 1), (256, 256, 4096)]
+    cplx = dtype == np.complex128
+    alpha = (0.75 - 0.5j) if cplx else 0.75
+    beta = (0.5 + 0.25j) if cplx else -0.5
+    specs, host = make_blocks(7, sizes, dtype, device, pad=(3, 1, 2))
+    batch = spla.Batch(ctx, specs, alpha, beta, complex_=cplx, opA=opA)
+    batch.execute()
+    for s, h in zip(specs, host):
+        check_block(s, h, opA, alpha, beta)
+    info = batch.info()
+    assert info["items"] >= 1 and info["workgroups"] >= 1 and info["model_ms"] > 0
+    batch.close()
+
+
+def test_batch_beta_zero_ignores_nan_in_c(ctx, device):
+    sizes = [(70, 50, 300), (130, 90, 1000)]
+    specs, host = make_blocks(8, sizes, np.complex128, device)
+    for s in specs:
+        s["C"].fill_(complex(float("nan"), 0.0))
+    batch = spla.Batch(ctx, specs, 1.0, 0.0)
+    batch.execute()
+    for s, (A, B, _) in zip(specs, host):
+        m, n, k = s["m"], s["n"], s["k"]
+        ref = np.zeros(s["ldc"] * n, dtype=np.complex128)
+        oracle.gemm(2, 0, m, n, k, 1.0, A, s["lda"], B, s["ldb"], 0.0, ref, s["ldc"], accurate=True)
+        got = to_matrix(s["C"].cpu().numpy(), m, n, s["ldc"])
+        assert rel_error(got, to_matrix(ref, m, n, s["ldc"]), A, B) <= 1e-13
+
+
+@pytest.mark.parametrize("fill", [spla.FillMode.UPPER, spla.FillMode.LOWER])
+def test_batch_fill_modes(ctx, device, fill):
+    # blocks of one triangular ring step: offsets place each block in C
+    fills, sizes = [], []
+    for bi in range(4):
+        for bj in range(4):
+            sizes.append((150, 130, 700))
+            fills.append((fill, bi * 150, bj * 130))
+    specs, host = make_blocks(9, sizes, np.complex128, device, fills=fills)
+    batch = spla.Batch(ctx, specs, 1.0 + 0.5j, 0.5)
+    batch.execute()
+    for s, h in zip(specs, host):
+        check_block(s, h, 2, 1.0 + 0.5j, 0.5)
+
+
+def test_batch_split_k_deterministic_over_replays(ctx, device):
+    import torch
+
+    sizes = [(256, 192, 60000), (64, 64, 90000), (300, 40, 5000), (96, 128, 30000)]
+    specs, host = make_blocks(10, sizes, np.complex128, device)
+    batch = spla.Batch(ctx, specs, 1.0, 0.0)
+    info = batch.info()
+    first = None
+    for rep in range(4):
+        # unrelated GPU work between replays
+        x = torch.randn(1 << 20, device=device)
+        torch.cuda.synchronize()
+        batch.execute()
+        out = [s["C"].cpu().numpy() for s in specs]
+        assert float(x.sum()) == float(x.sum())
+        if first is None:
+            first = out
+            for s, h in zip(specs, host):
+                check_block(s, h, 2, 1.0, 0.0)
+        else:
+            for a, b in zip(first, out):
+                assert np.array_equal(a, b), rep
+    # split-K in use: more items than output tiles
+    assert info["items"] > len(sizes)
+
+
+def test_batch_beta_accumulates_over_replays(ctx, device):
+    sizes = [(100, 80, 3000), (33, 65, 20000)]
+    specs, host = make_blocks(11, sizes, np.float64, device)
+    batch = spla.Batch(ctx, specs, 0.5, 1.0, complex_=False, opA=1)
+    for _ in range(3):
+        batch.execute()
+    for s, (A, B, C) in zip(specs, host):
+        ref = C.copy()
+        for _ in range(3):
+            oracle.gemm(1, 0, s["m"], s["n"], s["k"], 0.5, A, s["lda"], B, s["ldb"], 1.0, ref,
+                        s["ldc"], accurate=True)
+        m, n, ldc = s["m"], s["n"], s["ldc"]
+        got = to_matrix(s["C"].cpu().numpy(), m, n, ldc)
+        assert rel_error(got, to_matrix(ref, m, n, ldc), A, B) <= 3e-13
+
+
+def test_batch_measured_tuning_matches_model(ctx, device):
+    sizes = [(512, 512, 8000)] * 3 + [(128, 64, 20000)] * 5
+    specs, host = make_blocks(12, sizes, np.complex128, device)
+    specs2, _ = make_blocks(12, sizes, np.complex128, device)
+    b1 = spla.Batch(ctx, specs, 1.0, 0.0, measure=True)
+    b2 = spla.Batch(ctx, specs2, 1.0, 0.0)
+    assert b1.info()["measured_ms"] > 0
+    # tuning left C untouched
+    for s, (_, _, C) in zip(specs, host):
+        assert np.array_equal(s["C"].cpu().numpy(), C)
+    b1.execute()
+    b2.execute()
+    for s, s2, h in zip(specs, specs2, host):
+        check_block(s, h, 2, 1.0, 0.0)
+        check_block(s2, h, 2, 1.0, 0.0)
+
+
+def test_batch_split_precision(device):
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_gemm_mode(spla.GemmMode.SPLIT_INT8)
+    sizes = [(200, 150, 3000), (64, 300, 9000)]
+    specs, host = make_blocks(13, sizes, np.complex128, device)
+    batch = spla.Batch(ctx, specs, 1.0 - 1.0j, 0.25)
+    batch.execute()
+    assert ctx.last_product_info()["split"]
+    for s, h in zip(specs, host):
+        check_block(s, h, 2, 1.0 - 1.0j, 0.25)
+
+
+def test_batch_rejects_host_pointers(ctx, device):
+    A = np.zeros(64, dtype=np.complex128)
+    specs, _ = make_blocks(14, [(8, 8, 8)], np.complex128, device)
+    specs[0]["A"] = A
+    with pytest.raises(spla.GPUInvalidDevicePointerError):
+        spla.Batch(ctx, specs, 1.0, 0.0)
