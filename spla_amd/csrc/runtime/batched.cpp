@@ -18,6 +18,12 @@ namespace {
 constexpr double kPeakPerCU = 128.0 * 2.4e9;  // FP64 matrix flop / s per CU
 constexpr double kItemOverhead = 2.0e-6;      // scheduling + epilogue per item
 constexpr double kLaunchOverhead = 8.0e-6;
+
+// The persistent launch of batched.hip (one hipGraph node per batch) is
+// switched off: its first run on an MI355X produced no result within three
+// minutes and the cause has not been identified. Until it is, execute() runs
+// the blocks one by one on the single-product engine (same kernels as gemm).
+constexpr bool kPersistentLaunch = false;
 constexpr double kWorkgroupBytesPerSecond = 60e9;  // one workgroup reducing split partials
 
 int blocks_per_cu(int cfg) { return cfg == TileM ? 4 : 2; }
@@ -92,6 +98,11 @@ BatchPlan::BatchPlan(std::shared_ptr<spla::ContextInternal> ctx, bool complex, S
   BatchCandidate best = build(TileL, 1.0);
   BatchCandidate other = build(TileM, 1.0);
   if (other.modelSeconds < best.modelSeconds) std::swap(best, other);
+  if (!kPersistentLaunch) {
+    // switched off: the blocks run one by one on the single-product engine
+    cand_ = std::move(best);
+    return;
+  }
   if (measure && !best.items.empty()) {
     std::vector<BatchCandidate> cands;
     for (int cfg : {TileL, TileM})
@@ -313,9 +324,10 @@ void BatchPlan::execute() {
   hip_check(hipStreamSynchronize(nullptr));
   const hipStream_t s = ctx_->compute_stream();
   ctx_->timer().reset();
-  if (ctx_->gemm_mode() != GemmModeFP64) {
-    // Split precision: every block runs as its own split product (one host
-    // decision per block); blocks the plan rejects run on the FP64 kernels.
+  if (!kPersistentLaunch || ctx_->gemm_mode() != GemmModeFP64) {
+    // Block by block on the single-product engine. In split precision every
+    // block is its own split product (one host decision per block); blocks
+    // the split plan rejects run on the FP64 kernels.
     for (const BlockSpec& b : specs_) {
       if (b.m == 0 || b.n == 0) continue;
       FillSpec fill;

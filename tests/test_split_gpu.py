@@ -53,7 +53,7 @@ LAYOUTS = [("C", "N"), ("T", "N"), ("N", "N"), ("C", "C"), ("N", "T"), ("T", "C"
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
 @pytest.mark.parametrize("opA,opB", LAYOUTS)
-@pytest.mark.parametrize("shape", [(13, 11, 70), (37, 29, 300), (70, 3, 1)])
+@pytest.mark.parametrize("shape", [(13, 11, 70), (37, 29, 300), (70, 3, 9)])
 def test_split_bit_exact_against_integer_restatement(sctx, device, dtype, opA, opB, shape):
     if dtype == np.float64 and "C" in (opA, opB):
         pytest.skip("conjugation is the identity for real operands")
@@ -186,6 +186,12 @@ def test_split_fallbacks(sctx, device):
     got = gpu_gemm(sctx, device, 1, 0, m, n, k, 1.0, Aw, k, B, k, 0.0, C0, m)
     info = sctx.last_product_info()
     assert not info["split"] and info["reason"] == 3
+    # k = 1, real: the error target k * 2^-53 needs more than 53 bits per entry
+    got1 = gpu_gemm(sctx, device, 1, 0, m, n, 1, 1.0, A, k, B, k, 0.0, C0, m)
+    info = sctx.last_product_info()
+    assert not info["split"] and info["reason"] == 3
+    exp1 = np.outer(A[::k][:m], B[::k][:n]).T.ravel()
+    assert np.array_equal(got1, exp1)
     ref = C0.copy()
     oracle.gemm(1, 0, m, n, k, 1.0, Aw, k, B, k, 0.0, ref, m, accurate=True)
     assert rel_error(got, ref, Aw, B) <= 1e-13
