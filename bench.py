@@ -249,9 +249,9 @@ def main() -> int:
     best_form = max(ok, key=lambda f: ok[f]["tflops"]) if ok else "plain"
     best = forms[best_form]
     if best_form == "split" and best.get("split_used"):
-        # dominant kernel: the batched INT8 GEMM; executed ops = 2 m n kpad per product
-        kpad = (k + 127) // 128 * 128
-        ops = 2.0 * m * n * kpad * 3 * best["moduli"]
+        # dominant kernel: the batched INT8 GEMM; algorithmic ops = 2 m n k per product
+        # (3 complex sub-products per modulus; the zero padding of k is not counted)
+        ops = 2.0 * m * n * k * 3 * best["moduli"]
         gemm_ms = best["phase_ms_per_step"].get("split_int8_gemm", float("nan"))
         achieved = ops / (gemm_ms / 1e3) / 1e12
         roofline = {"bound": "mfma", "kernel": "i8_gemm_kernel (v_mfma_i32_32x32x32_i8)",

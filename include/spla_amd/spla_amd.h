@@ -59,7 +59,8 @@ SPLA_EXPORT SplaError spla_amd_ctx_set_split_tolerance(SplaContext ctx, double t
  * Outcome of the context's last double / complex-double product: used = 1 if
  * it ran in split precision; reason = 0 when used, else why FP64 ran
  * (1 non-finite input, 2 zero operand, 3 exponent range too wide, 4 too many
- * moduli, 5 norm out of range, 6 FP64 expected faster, 7 split mode off);
+ * moduli, 5 norm or reduction length out of range, 6 FP64 expected faster, 7 split
+ * mode off);
  * moduli, bitsA and bitsB describe the plan; bound is its estimated normwise
  * error bound. Any output pointer may be NULL.
  */
@@ -73,11 +74,12 @@ SPLA_EXPORT SplaError spla_amd_ctx_last_product_info(SplaContext ctx, int* used,
  * One ring step of pgemm_ssb / pgemm_ssbtr forms many independent block
  * products C_b <- alpha op(A_b) B_b + beta C_b with op(A) = A^T or A^H
  * (SPLA src/pgemm_ssb/ring_ssb_gpu.cpp:193-197 issues one vendor GEMM per
- * block). A batch runs all of them in one persistent launch captured in a
- * hipGraph; tile shape and per-block split-K factors come from the library's
- * autotuner. All pointers must be device memory. A block's fill mode and
- * offsets follow pgemm_ssbtr: tiles wholly outside the requested triangle of
- * the global C are skipped.
+ * block). A batch is planned for one persistent launch (tile shape and
+ * per-block split-K factors from a cost model of the persistent schedule);
+ * in this build that launch is switched off and the blocks run one after the
+ * other on the single-product engine, with the same results. All pointers
+ * must be device memory. A block's fill mode and offsets follow pgemm_ssbtr:
+ * tiles wholly outside the requested triangle of the global C are skipped.
  */
 typedef void* SplaAmdBatch;
 
@@ -115,7 +117,7 @@ SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext
                                                 const void* beta, SplaAmdTuning tuning);
 
 /* Runs the batch (synchronous). In split-precision mode every block runs as
- * its own split product instead of the captured FP64 launch. */
+ * its own split product. */
 SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch);
 
 /* Plan summary: tile configuration (0: 128 x 128, 1: 64 x 64 embedded), work
@@ -125,6 +127,57 @@ SPLA_EXPORT SplaError spla_amd_batch_info(SplaAmdBatch batch, int* tileConfig, i
                                           int* workgroups, double* modelMs, double* measuredMs);
 
 SPLA_EXPORT SplaError spla_amd_batch_destroy(SplaAmdBatch* batch);
+
+/* ------------------------------------------------------------------------
+ * Ring-step planning (host only, no device work).
+ *
+ * The blocks a pgemm_ssb / pgemm_ssbtr ring computes and where each piece of
+ * a block lives, by SPLA's own rules (block shape
+ * src/util/block_size_selection.hpp:57-186 with the constants of
+ * src/pgemm_ssb/pgemm_ssb_gpu.cpp:108-112, block order :170-190, fill
+ * predicate src/block_generation/block.hpp:58-66, placement
+ * block_cyclic_generator.cpp:38-112 and mirror_generator.hpp:40-110). The
+ * block products of one ring step are a batch (spla_amd_batch_create_ssb).
+ */
+typedef struct SplaAmdRingBlock {
+  int row, col;          /* inside the m x n sub-matrix of C */
+  int numRows, numCols;
+} SplaAmdRingBlock;
+
+typedef struct SplaAmdPiece {
+  int globalRow, globalCol;  /* first element in the global C */
+  int subRow, subCol;        /* relative to the region's origin */
+  int localRow, localCol;    /* first element in the owner's local C */
+  int numRows, numCols;
+  int rank;                  /* owner, column-major over the grid; -1: every rank */
+} SplaAmdPiece;
+
+#ifndef __cplusplus
+typedef struct SplaAmdRingBlock SplaAmdRingBlock;
+typedef struct SplaAmdPiece SplaAmdPiece;
+#endif
+
+/* Block shape and ordered active blocks of pgemm_ssb / ssbtr on a distribution
+ * of C (type, process grid) over commSize ranks; consecutive groups of
+ * commSize blocks form one ring tile. numBlocks receives the total; at most
+ * capacity blocks are written (blocks may be NULL). */
+SPLA_EXPORT SplaError spla_amd_ssb_ring_blocks(SplaDistributionType distType, int gridRows,
+                                               int gridCols, int commSize, int m, int n,
+                                               int cRowOffset, int cColOffset,
+                                               SplaFillMode cFillMode, int tileSizeHost,
+                                               int* rowsInBlock, int* colsInBlock,
+                                               int* numBlocks, SplaAmdRingBlock* blocks,
+                                               int capacity);
+
+/* Pieces of the region numRows x numCols at (rowOffset, colOffset) of C:
+ * block-cyclic blocks of rowsInBlock x colsInBlock on a gridRows x gridCols
+ * grid, or (mirror) tiles of that size held by every rank. numPieces receives
+ * the total; at most capacity pieces are written (pieces may be NULL). */
+SPLA_EXPORT SplaError spla_amd_region_pieces(SplaDistributionType distType, int rowsInBlock,
+                                             int colsInBlock, int gridRows, int gridCols,
+                                             int numRows, int numCols, int rowOffset,
+                                             int colOffset, int* numPieces, SplaAmdPiece* pieces,
+                                             int capacity);
 
 /* Kernel phases reported by spla_amd_ctx_last_timing. */
 enum SplaAmdPhase {

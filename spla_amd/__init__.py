@@ -38,4 +38,6 @@ from spla_amd.api import (  # noqa: F401
     pgemm_sbs,
     pgemm_ssb,
     pgemm_ssbtr,
+    region_pieces,
+    ssb_ring_blocks,
 )

@@ -35,6 +35,17 @@ class SplaAmdBlock(ctypes.Structure):
                 ("fillMode", c_int), ("rowOffset", c_int), ("colOffset", c_int)]
 
 
+class SplaAmdRingBlock(ctypes.Structure):
+    """struct SplaAmdRingBlock of include/spla_amd/spla_amd.h."""
+    _fields_ = [("row", c_int), ("col", c_int), ("numRows", c_int), ("numCols", c_int)]
+
+
+class SplaAmdPiece(ctypes.Structure):
+    """struct SplaAmdPiece of include/spla_amd/spla_amd.h."""
+    _fields_ = [(f, c_int) for f in ("globalRow", "globalCol", "subRow", "subCol", "localRow",
+                                     "localCol", "numRows", "numCols", "rank")]
+
+
 _lib = None
 
 
@@ -100,6 +111,10 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("spla_amd_batch_execute", c_void_p)
     fn("spla_amd_batch_info", c_void_p, P(c_int), P(c_int), P(c_int), P(c_double), P(c_double))
     fn("spla_amd_batch_destroy", P(c_void_p))
+    fn("spla_amd_ssb_ring_blocks", c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+       c_int, P(c_int), P(c_int), P(c_int), c_void_p, c_int)
+    fn("spla_amd_region_pieces", c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+       P(c_int), c_void_p, c_int)
     fn("spla_amd_ctx_last_product_info", c_void_p, P(c_int), P(c_int), P(c_int), P(c_int),
        P(c_int), P(c_double))
 

@@ -367,6 +367,33 @@ class Batch:
             pass
 
 
+def ssb_ring_blocks(dist_type, grid_rows, grid_cols, comm_size, m, n, row_offset, col_offset,
+                    fill, tile_size_host):
+    """spla_amd_ssb_ring_blocks: (rows_in_block, cols_in_block, [(row, col, rows, cols)])."""
+    lib = _lib.lib()
+    rib, cib, cnt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    args = (int(dist_type), grid_rows, grid_cols, comm_size, m, n, row_offset, col_offset,
+            int(fill), tile_size_host, ctypes.byref(rib), ctypes.byref(cib), ctypes.byref(cnt))
+    check(lib.spla_amd_ssb_ring_blocks(*args, None, 0))
+    arr = (_lib.SplaAmdRingBlock * max(1, cnt.value))()
+    check(lib.spla_amd_ssb_ring_blocks(*args, arr, cnt.value))
+    return rib.value, cib.value, [(b.row, b.col, b.numRows, b.numCols) for b in arr[:cnt.value]]
+
+
+def region_pieces(dist_type, rows_in_block, cols_in_block, grid_rows, grid_cols, rows, cols,
+                  row_offset, col_offset):
+    """spla_amd_region_pieces: list of 9-tuples (globalRow, globalCol, subRow, subCol,
+    localRow, localCol, numRows, numCols, rank)."""
+    lib = _lib.lib()
+    cnt = ctypes.c_int()
+    args = (int(dist_type), rows_in_block, cols_in_block, grid_rows, grid_cols, rows, cols,
+            row_offset, col_offset, ctypes.byref(cnt))
+    check(lib.spla_amd_region_pieces(*args, None, 0))
+    arr = (_lib.SplaAmdPiece * max(1, cnt.value))()
+    check(lib.spla_amd_region_pieces(*args, arr, cnt.value))
+    return [tuple(getattr(q, f) for f, _ in _lib.SplaAmdPiece._fields_) for q in arr[:cnt.value]]
+
+
 # ---------------------------------------------------------------------------
 # array plumbing
 

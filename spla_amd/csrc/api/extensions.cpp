@@ -1,6 +1,10 @@
 // Entry points of include/spla_amd/spla_amd.h (no SPLA counterpart).
+#include <algorithm>
+#include <cstddef>
+
 #include "api/internal_access.hpp"
 #include "runtime/batched.hpp"
+#include "runtime/ring_plan.hpp"
 #include "spla_amd/spla_amd.h"
 
 extern "C" {
@@ -109,6 +113,63 @@ SPLA_EXPORT SplaError spla_amd_batch_destroy(SplaAmdBatch* batch) {
   SPLA_C_GUARD({
     delete static_cast<spla_amd::BatchPlan*>(*batch);
     *batch = nullptr;
+  })
+}
+
+SPLA_EXPORT SplaError spla_amd_ssb_ring_blocks(SplaDistributionType distType, int gridRows,
+                                               int gridCols, int commSize, int m, int n,
+                                               int cRowOffset, int cColOffset,
+                                               SplaFillMode cFillMode, int tileSizeHost,
+                                               int* rowsInBlock, int* colsInBlock,
+                                               int* numBlocks, SplaAmdRingBlock* blocks,
+                                               int capacity) {
+  if ((distType != SPLA_DIST_BLACS_BLOCK_CYCLIC && distType != SPLA_DIST_MIRROR) ||
+      gridRows < 1 || gridCols < 1 || commSize < 1 || m < 0 || n < 0 || cRowOffset < 0 ||
+      cColOffset < 0 || tileSizeHost < 1 || capacity < 0 ||
+      (cFillMode != SPLA_FILL_MODE_FULL && cFillMode != SPLA_FILL_MODE_UPPER &&
+       cFillMode != SPLA_FILL_MODE_LOWER))
+    return SPLA_INVALID_PARAMETER_ERROR;
+  if (!numBlocks || (capacity > 0 && !blocks)) return SPLA_INVALID_POINTER_ERROR;
+  SPLA_C_GUARD({
+    const auto p = spla_amd::plan_ssb_ring(distType == SPLA_DIST_MIRROR, gridRows, gridCols,
+                                           commSize, m, n, cRowOffset, cColOffset, cFillMode,
+                                           tileSizeHost);
+    if (rowsInBlock) *rowsInBlock = static_cast<int>(p.rowsInBlock);
+    if (colsInBlock) *colsInBlock = static_cast<int>(p.colsInBlock);
+    *numBlocks = static_cast<int>(p.blocks.size());
+    const std::size_t w = std::min<std::size_t>(p.blocks.size(), static_cast<std::size_t>(capacity));
+    for (std::size_t i = 0; i < w; ++i) {
+      const auto& b = p.blocks[i];
+      blocks[i] = SplaAmdRingBlock{static_cast<int>(b.row), static_cast<int>(b.col),
+                                   static_cast<int>(b.rows), static_cast<int>(b.cols)};
+    }
+  })
+}
+
+SPLA_EXPORT SplaError spla_amd_region_pieces(SplaDistributionType distType, int rowsInBlock,
+                                             int colsInBlock, int gridRows, int gridCols,
+                                             int numRows, int numCols, int rowOffset,
+                                             int colOffset, int* numPieces, SplaAmdPiece* pieces,
+                                             int capacity) {
+  if ((distType != SPLA_DIST_BLACS_BLOCK_CYCLIC && distType != SPLA_DIST_MIRROR) ||
+      rowsInBlock < 1 || colsInBlock < 1 || gridRows < 1 || gridCols < 1 || numRows < 0 ||
+      numCols < 0 || rowOffset < 0 || colOffset < 0 || capacity < 0)
+    return SPLA_INVALID_PARAMETER_ERROR;
+  if (!numPieces || (capacity > 0 && !pieces)) return SPLA_INVALID_POINTER_ERROR;
+  SPLA_C_GUARD({
+    const auto v = spla_amd::region_pieces(distType == SPLA_DIST_MIRROR, rowsInBlock, colsInBlock,
+                                           gridRows, gridCols, numRows, numCols, rowOffset,
+                                           colOffset);
+    *numPieces = static_cast<int>(v.size());
+    const std::size_t w = std::min<std::size_t>(v.size(), static_cast<std::size_t>(capacity));
+    for (std::size_t i = 0; i < w; ++i) {
+      const auto& q = v[i];
+      pieces[i] = SplaAmdPiece{static_cast<int>(q.globalRow), static_cast<int>(q.globalCol),
+                               static_cast<int>(q.subRow),    static_cast<int>(q.subCol),
+                               static_cast<int>(q.localRow),  static_cast<int>(q.localCol),
+                               static_cast<int>(q.rows),      static_cast<int>(q.cols),
+                               static_cast<int>(q.rank)};
+    }
   })
 }
 

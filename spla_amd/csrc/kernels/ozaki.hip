@@ -5,12 +5,12 @@
 //   stats_*      per-row largest |component|, sum of squares, non-finite flag
 //                (HBM bound: reads each operand once);
 //   residue_*    scaled integer rows -> symmetric residues modulo every m_t,
-//                written as K-major int8 planes padded to 128 bytes
+//                written as K-major int8 planes padded to 256 bytes
 //                (HBM bound: reads the operand once, writes N*products bytes
 //                per element);
 //   i8_gemm      batched C_g = R_A,g R_B,g^T over all moduli and complex
 //                sub-products in one launch, INT32 accumulation; splits of at
-//                most 2^17 - 128 reduction bytes keep the sums below 2^31
+//                most 2^17 - 256 reduction bytes keep the sums below 2^31
 //                (|residue| <= 128). One split: the epilogue reduces modulo
 //                m_t and stores uint8; several splits: raw int32 sums;
 //   crt          per output element: modular recombination of the complex
@@ -286,6 +286,14 @@ __device__ __forceinline__ bool tile_active(int mode, int64_t rowOff, int64_t co
   return true;
 }
 
+// Residue planes hold whole tiles: rows are padded to a multiple of the tile
+// height (256) so that every load of a tile is in bounds without masks; padded
+// rows only feed output rows / columns that are never stored.
+//
+// Pipeline per k-block (BK = 128 bytes): global loads run two k-blocks ahead
+// through two register sets, LDS is double buffered (one barrier per k-block)
+// and the MFMA operands of the next 32-byte step are read from LDS while the
+// current step's 16 MFMAs issue.
 template <bool OUT8, int WM, int WN>
 __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   constexpr int BM = 2 * WM;
@@ -294,8 +302,8 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   constexpr int kStride = BK + 16;
   constexpr int FM = WM / 32;
   constexpr int FN = WN / 32;
-  constexpr int CA = BM * (BK / 16) / 256;  // 16-byte chunks per thread
-  constexpr int CB = BN * (BK / 16) / 256;
+  constexpr int CA = BM / 32;  // 16-byte chunks per thread and k-block (32 rows per pass)
+  constexpr int CB = BN / 32;
   __shared__ __attribute__((aligned(16))) int8_t lA[2][BM][kStride];
   __shared__ __attribute__((aligned(16))) int8_t lB[2][BN][kStride];
 
@@ -325,10 +333,18 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   const int64_t kBegin = static_cast<int64_t>(s) * d.kSplit;
   const int64_t kEnd = min(d.kpad, kBegin + d.kSplit);
   const int nk = static_cast<int>((kEnd - kBegin) / BK);
-  const int8_t* Ab = d.RA + static_cast<int64_t>(g) * d.planeA;
-  const int8_t* Bb = d.RB + static_cast<int64_t>(g) * d.planeB;
-
+  const int64_t kpad = d.kpad;
+  // tile origins (uniform) and the thread's offset inside a 32-row pass
+  const int8_t* At = d.RA + static_cast<int64_t>(g) * d.planeA + static_cast<int64_t>(row0) * kpad;
+  const int8_t* Bt = d.RB + static_cast<int64_t>(g) * d.planeB + static_cast<int64_t>(col0) * kpad;
   const int tid = threadIdx.x;
+  // byte offsets of the thread's 16-byte chunks inside a tile (same for A and B)
+  static_assert(CA == CB, "A and B tiles load the same rows per thread");
+  uint32_t voff[CA];
+#pragma unroll
+  for (int u = 0; u < CA; ++u)
+    voff[u] = static_cast<uint32_t>(((tid >> 3) + 32 * u) * kpad + 16 * (tid & 7));
+
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int wm = wid & 1;
@@ -344,66 +360,80 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[f][q][r] = 0;
 
-  int4v ra[CA], rb[CB];
-  auto load = [&](int64_t k0) {
+  int4v ra0[CA], rb0[CB], ra1[CA], rb1[CB];
+  // k-block index kb is clamped to the last one: the surplus loads of the
+  // pipeline tail re-read it and are never used
+  auto load = [&](int4v (&ra)[CA], int4v (&rb)[CB], int kb) {
+    const int64_t k0 = kBegin + static_cast<int64_t>(min(kb, nk - 1)) * BK;
+    const int8_t* Ak = At + k0;
+    const int8_t* Bk = Bt + k0;
 #pragma unroll
-    for (int u = 0; u < CA; ++u) {
-      const int c = tid + u * 256;
-      const int row = c >> 3, seg = c & 7;
-      ra[u] = int4v{0, 0, 0, 0};
-      if (row0 + row < d.m)
-        ra[u] = *reinterpret_cast<const int4v*>(Ab + static_cast<int64_t>(row0 + row) * d.kpad +
-                                                k0 + 16 * seg);
-    }
+    for (int u = 0; u < CA; ++u) ra[u] = *reinterpret_cast<const int4v*>(Ak + voff[u]);
 #pragma unroll
-    for (int u = 0; u < CB; ++u) {
-      const int c = tid + u * 256;
-      const int row = c >> 3, seg = c & 7;
-      rb[u] = int4v{0, 0, 0, 0};
-      if (col0 + row < d.n)
-        rb[u] = *reinterpret_cast<const int4v*>(Bb + static_cast<int64_t>(col0 + row) * d.kpad +
-                                                k0 + 16 * seg);
-    }
+    for (int u = 0; u < CB; ++u) rb[u] = *reinterpret_cast<const int4v*>(Bk + voff[u]);
   };
-  auto store = [&](int buf) {
+  auto store = [&](const int4v (&ra)[CA], const int4v (&rb)[CB], int buf) {
 #pragma unroll
-    for (int u = 0; u < CA; ++u) {
-      const int c = tid + u * 256;
-      *reinterpret_cast<int4v*>(&lA[buf][c >> 3][16 * (c & 7)]) = ra[u];
-    }
+    for (int u = 0; u < CA; ++u)
+      *reinterpret_cast<int4v*>(&lA[buf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = ra[u];
 #pragma unroll
-    for (int u = 0; u < CB; ++u) {
-      const int c = tid + u * 256;
-      *reinterpret_cast<int4v*>(&lB[buf][c >> 3][16 * (c & 7)]) = rb[u];
-    }
+    for (int u = 0; u < CB; ++u)
+      *reinterpret_cast<int4v*>(&lB[buf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = rb[u];
   };
-
-  if (nk > 0) {
-    load(kBegin);
-    store(0);
-  }
-  __syncthreads();
-  for (int it = 0; it < nk; ++it) {
-    const int cur = it & 1;
-    const bool more = it + 1 < nk;
-    if (more) load(kBegin + static_cast<int64_t>(it + 1) * BK);
+  auto compute = [&](int buf) {
+    int4v a[2][FM], b[2][FN];
+#pragma unroll
+    for (int f = 0; f < FM; ++f)
+      a[0][f] = *reinterpret_cast<const int4v*>(&lA[buf][wm * WM + 32 * f + l32][16 * khalf]);
+#pragma unroll
+    for (int q = 0; q < FN; ++q)
+      b[0][q] = *reinterpret_cast<const int4v*>(&lB[buf][wn * WN + 32 * q + l32][16 * khalf]);
 #pragma unroll
     for (int ks = 0; ks < BK / 32; ++ks) {
-      int4v a[FM], b[FN];
+      const int cur = ks & 1;
+      if (ks + 1 < BK / 32) {
 #pragma unroll
-      for (int f = 0; f < FM; ++f)
-        a[f] = *reinterpret_cast<const int4v*>(&lA[cur][wm * WM + 32 * f + l32][ks * 32 + 16 * khalf]);
+        for (int f = 0; f < FM; ++f)
+          a[cur ^ 1][f] = *reinterpret_cast<const int4v*>(
+              &lA[buf][wm * WM + 32 * f + l32][(ks + 1) * 32 + 16 * khalf]);
 #pragma unroll
-      for (int q = 0; q < FN; ++q)
-        b[q] = *reinterpret_cast<const int4v*>(&lB[cur][wn * WN + 32 * q + l32][ks * 32 + 16 * khalf]);
+        for (int q = 0; q < FN; ++q)
+          b[cur ^ 1][q] = *reinterpret_cast<const int4v*>(
+              &lB[buf][wn * WN + 32 * q + l32][(ks + 1) * 32 + 16 * khalf]);
+      }
+      // the next step's LDS reads stay ahead of this step's MFMAs
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int f = 0; f < FM; ++f)
 #pragma unroll
         for (int q = 0; q < FN; ++q)
-          acc[f][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[f], b[q], acc[f][q], 0, 0, 0);
+          acc[f][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cur][f], b[cur][q], acc[f][q], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (more) store(cur ^ 1);
+  };
+
+  // nk is even (splits are whole multiples of 2 * BK) and positive for every
+  // launched block; the test only guards the address arithmetic
+  if (nk > 0) {
+    load(ra0, rb0, 0);
+    store(ra0, rb0, 0);
+    load(ra1, rb1, 1);
     __syncthreads();
+    for (int it = 0; it < nk; it += 2) {
+      // even step: LDS buffer 0 holds k-block it, register set 1 holds it + 1;
+      // the scheduling barriers keep the global loads ahead of the MFMAs
+      load(ra0, rb0, it + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(0);
+      store(ra1, rb1, 1);
+      __syncthreads();
+      // odd step: LDS buffer 1 holds k-block it + 1, register set 0 holds it + 2
+      load(ra1, rb1, it + 3);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(1);
+      store(ra0, rb0, 0);
+      __syncthreads();
+    }
   }
 
   // D layout of v_mfma_i32_32x32x32_i8: lane holds column lane % 32 and rows

@@ -56,7 +56,7 @@ struct I8GemmDesc {
   int G;         // number of products (moduli * products per modulus)
   int products;  // 1 (real) or 3 (complex)
   int splits;
-  int64_t kSplit;  // reduction bytes per split (multiple of 128)
+  int64_t kSplit;  // reduction bytes per split (multiple of 256)
   int tilesM, tilesN;
   void* out;       // uint8 residues [G][n][m] (splits == 1) or int32 sums [splits][G][n][m]
   int64_t outPlane;  // m * n

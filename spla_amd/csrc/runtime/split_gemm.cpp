@@ -14,10 +14,16 @@
 namespace spla_amd {
 namespace {
 
-constexpr int64_t kKpadAlign = 128;
+// Residue rows and reduction splits are whole multiples of two GEMM k-blocks
+// (2 x 128 bytes): the GEMM pipeline runs k-blocks in pairs.
+constexpr int64_t kKpadAlign = 256;
 // Largest split of the reduction (bytes of int8 residues) whose INT32 sums
-// cannot overflow: 128 * 128 * (2^17 - 128) < 2^31.
-constexpr int64_t kMaxSplitK = (int64_t(1) << 17) - 128;
+// cannot overflow: 128 * 128 * (2^17 - 256) < 2^31.
+constexpr int64_t kMaxSplitK = (int64_t(1) << 17) - 256;
+// longest padded residue row: offsets of the 256 rows of a tile fit 32 bits
+constexpr int64_t kMaxKpad = int64_t(1) << 24;
+// residue planes hold whole tiles of the INT8 GEMM (rows padded to 256)
+constexpr int64_t kPlaneRowAlign = 256;
 // Sustained rates assumed by the automatic mode.
 constexpr double kInt8OpsPerSecond = 2.0e15;
 constexpr double kHbmBytesPerSecond = 4.0e12;
@@ -56,6 +62,10 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   const int mode = ctx.gemm_mode();
   if (mode == GemmModeFP64 || k == 0 || m == 0 || n == 0) {
     rep.reason = SplitDisabled;
+    return false;
+  }
+  if (round_up(k, kKpadAlign) > kMaxKpad) {  // residue rows address with 32-bit offsets
+    rep.reason = SplitRange;
     return false;
   }
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
@@ -130,9 +140,10 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   const CrtConstants* dCC = reinterpret_cast<const CrtConstants*>(dUp + expBytes);
 
   // 4. residue planes
-  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(G) * m * kpad);
+  const int64_t mpad = round_up(m, kPlaneRowAlign), npad = round_up(n, kPlaneRowAlign);
+  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(G) * mpad * kpad);
   int8_t* RA = keep.back().get<int8_t>();
-  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(G) * n * kpad);
+  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(G) * npad * kpad);
   int8_t* RB = keep.back().get<int8_t>();
   ResidueDesc rd;
   std::memset(&rd, 0, sizeof(rd));
@@ -144,7 +155,7 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   rd.expo = dExpA;
   rd.imagSign = (cplx && opA == SPLA_OP_CONJ_TRANSPOSE) ? -1 : 1;
   rd.R = RA;
-  rd.plane = m * kpad;
+  rd.plane = mpad * kpad;
   rd.ms = ms;
   if (timing) timer.begin(PhaseSplitResidues, stream);
   hip_check(launch_residues(cplx, la, rd, stream));
@@ -154,7 +165,7 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   rd.expo = dExpB;
   rd.imagSign = (cplx && opB == SPLA_OP_CONJ_TRANSPOSE) ? -1 : 1;
   rd.R = RB;
-  rd.plane = n * kpad;
+  rd.plane = npad * kpad;
   hip_check(launch_residues(cplx, lb, rd, stream));
   if (timing) timer.end(stream);
 
@@ -177,8 +188,8 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   std::memset(&gd, 0, sizeof(gd));
   gd.RA = RA;
   gd.RB = RB;
-  gd.planeA = m * kpad;
-  gd.planeB = n * kpad;
+  gd.planeA = mpad * kpad;
+  gd.planeB = npad * kpad;
   gd.kpad = kpad;
   gd.m = mi;
   gd.n = ni;
