@@ -163,6 +163,8 @@ def main() -> int:
     ap.add_argument("--forms", default="plain,split", help="comma list of: plain, split")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (reference kernel for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -192,6 +194,15 @@ def main() -> int:
     tiny = torch.zeros(64 * 64, dtype=torch.complex128, device=dev)
     spla.pgemm_ssb(64, 64, 128, spla.Operation.CONJ_TRANSPOSE, ALPHA, dA, k, dB, k, BETA, tiny,
                    64, 0, 0, dist, ctx)
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
 
     flops = 8.0 * m * n * k
     modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8}

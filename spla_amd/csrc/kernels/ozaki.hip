@@ -23,9 +23,15 @@
 
 #include "kernels/launch.hpp"
 #include "kernels/ozaki_desc.hpp"
+#include "kernels/residue_arith.hpp"
 
 namespace spla_amd {
 namespace {
+
+using residue::Pieces;
+using residue::pieces16;
+using residue::sym_fold;
+using residue::sym_mod4;
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 typedef int int16v __attribute__((ext_vector_type(16)));
@@ -140,79 +146,78 @@ __global__ void __launch_bounds__(256) stats_finalize_kernel(const double* pmax,
 // ---------------------------------------------------------------------------
 // residues
 
-// Symmetric residue of an exact integer a (|a| <= 2^54) modulo m:
-// [-(m/2), m - 1 - (m/2)], i.e. [-128, 127] for m = 256 and [-(m-1)/2, (m-1)/2] for odd m.
-__device__ __forceinline__ int sym_mod(double a, double m, double inv, double lo, double hi) {
-  const double q = rint(a * inv);
-  double r = fma(-q, m, a);  // exact: the true value is a small integer
-  r = (r > hi) ? r - m : r;
-  r = (r < lo) ? r + m : r;
-  return static_cast<int>(r);
-}
-
-__device__ __forceinline__ int sym_fold(int r, int m, int lo, int hi) {
-  r = (r > hi) ? r - m : r;
-  return (r < lo) ? r + m : r;
-}
-
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
   return (static_cast<uint32_t>(a) & 0xffu) | ((static_cast<uint32_t>(b) & 0xffu) << 8) |
          ((static_cast<uint32_t>(c) & 0xffu) << 16) | ((static_cast<uint32_t>(d) & 0xffu) << 24);
 }
 
-// Reduction index contiguous: one thread converts 16 consecutive elements of a row.
+// Reduction index contiguous: a block converts 1024 consecutive elements of a
+// row. The row segment is read with whole-wave contiguous 16-byte loads into
+// LDS, then every thread takes 4 consecutive elements and writes one 32-bit
+// word per residue plane (a wave writes 256 contiguous bytes per plane).
+constexpr int kResElems = 1024;  // elements per block
+__device__ __forceinline__ int res_slot(int e) { return e + (e >> 3); }  // conflict-free reads
+
 template <bool CPLX>
 __global__ void __launch_bounds__(256) residue_kmajor_kernel(ResidueDesc d) {
+  constexpr int C = CPLX ? 2 : 1;  // doubles per element
+  // 16-byte slots: one complex element, or two real elements
+  constexpr int kSlots = kResElems * C / 2;
+  __shared__ __attribute__((aligned(16))) double2 lds[kSlots + kSlots / 8];
   const int row = blockIdx.y;
-  const int64_t k0 = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) * 16;
-  if (row >= d.rows || k0 >= d.kpad) return;
-  const int e = d.expo[row];
-  const double* src = d.X + static_cast<int64_t>(row) * d.ld * (CPLX ? 2 : 1);
-  double re[16], im[16];
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kResElems;
+  const int tid = threadIdx.x;
+  const double* src = d.X + static_cast<int64_t>(row) * d.ld * C + e0 * C;
+  const int64_t avail = (d.k - e0) * C;  // doubles of the row left from e0 (may be <= 0)
+  const bool aligned = (reinterpret_cast<uintptr_t>(src) & 15) == 0;  // uniform per block
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int64_t l = k0 + j;
-    re[j] = 0.0;
-    im[j] = 0.0;
-    if (l < d.k) {
-      if (CPLX) {
-        re[j] = rint(ldexp(src[2 * l], e));
-        im[j] = d.imagSign * rint(ldexp(src[2 * l + 1], e));
-      } else {
-        re[j] = rint(ldexp(src[l], e));
-      }
+  for (int j = 0; j < kSlots / 256; ++j) {
+    const int slot = tid + 256 * j;
+    double2 v = {0.0, 0.0};
+    if (aligned && 2 * slot + 1 < avail) {
+      v = *reinterpret_cast<const double2*>(src + 2 * slot);
+    } else {
+      if (2 * slot < avail) v.x = src[2 * slot];
+      if (2 * slot + 1 < avail) v.y = src[2 * slot + 1];
+    }
+    lds[res_slot(slot)] = v;
+  }
+  __syncthreads();
+  const int64_t l0 = e0 + 4 * tid;  // first of this thread's 4 elements
+  if (l0 >= d.kpad) return;
+  const int e = d.expo[row];
+  Pieces re[4], im[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (CPLX) {
+      const double2 v = lds[res_slot(4 * tid + j)];
+      re[j] = pieces16(rint(ldexp(v.x, e)));
+      im[j] = pieces16(d.imagSign * rint(ldexp(v.y, e)));
+    } else {
+      const double2 v = lds[res_slot(2 * tid + (j >> 1))];
+      re[j] = pieces16(rint(ldexp((j & 1) ? v.y : v.x, e)));
+      im[j] = Pieces{0, 0, 0, 0};
     }
   }
   constexpr int P = CPLX ? 3 : 1;
+  int8_t* base = d.R + static_cast<int64_t>(row) * d.kpad + l0;
   for (int t = 0; t < d.ms.count; ++t) {
-    const int mi = d.ms.m[t];
-    const double m = mi, inv = d.ms.inv[t];
+    const int mi = d.ms.m[t], c16 = d.ms.c16[t], c32 = d.ms.c32[t], c48 = d.ms.c48[t];
+    const float invf = d.ms.invf[t];
     const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
-    int r0[16], r1[16];
+    int r0[4], r1[4];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      r0[j] = sym_mod(re[j], m, inv, lo, hi);
-      r1[j] = CPLX ? sym_mod(im[j], m, inv, lo, hi) : 0;
+    for (int j = 0; j < 4; ++j) {
+      r0[j] = sym_mod4(re[j], mi, c16, c32, c48, invf, lo, hi);
+      r1[j] = CPLX ? sym_mod4(im[j], mi, c16, c32, c48, invf, lo, hi) : 0;
     }
-    int8_t* base = d.R + static_cast<int64_t>(t * P) * d.plane + static_cast<int64_t>(row) * d.kpad + k0;
-    int4v w0;
-    w0[0] = pack4(r0[0], r0[1], r0[2], r0[3]);
-    w0[1] = pack4(r0[4], r0[5], r0[6], r0[7]);
-    w0[2] = pack4(r0[8], r0[9], r0[10], r0[11]);
-    w0[3] = pack4(r0[12], r0[13], r0[14], r0[15]);
-    *reinterpret_cast<int4v*>(base) = w0;
+    int8_t* p = base + static_cast<int64_t>(t * P) * d.plane;
+    *reinterpret_cast<uint32_t*>(p) = pack4(r0[0], r0[1], r0[2], r0[3]);
     if (CPLX) {
-      int4v w1, w2;
-      int r2[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) r2[j] = sym_fold(r0[j] + r1[j], mi, lo, hi);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        w1[q] = pack4(r1[4 * q], r1[4 * q + 1], r1[4 * q + 2], r1[4 * q + 3]);
-        w2[q] = pack4(r2[4 * q], r2[4 * q + 1], r2[4 * q + 2], r2[4 * q + 3]);
-      }
-      *reinterpret_cast<int4v*>(base + d.plane) = w1;
-      *reinterpret_cast<int4v*>(base + 2 * d.plane) = w2;
+      *reinterpret_cast<uint32_t*>(p + d.plane) = pack4(r1[0], r1[1], r1[2], r1[3]);
+      *reinterpret_cast<uint32_t*>(p + 2 * d.plane) =
+          pack4(sym_fold(r0[0] + r1[0], mi, lo, hi), sym_fold(r0[1] + r1[1], mi, lo, hi),
+                sym_fold(r0[2] + r1[2], mi, lo, hi), sym_fold(r0[3] + r1[3], mi, lo, hi));
     }
   }
 }
@@ -230,34 +235,35 @@ __global__ void __launch_bounds__(256) residue_mnmajor_kernel(ResidueDesc d) {
   const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64;
   const int row = row0 + tx;
   const int e = row < d.rows ? d.expo[row] : 0;
-  double re[16], im[16];
+  Pieces re[16], im[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int64_t l = k0 + ty + 4 * j;
-    re[j] = 0.0;
-    im[j] = 0.0;
+    double vr = 0.0, vi = 0.0;
     if (row < d.rows && l < d.k) {
       if (CPLX) {
         const double* p = d.X + (l * d.ld + row) * 2;
-        re[j] = rint(ldexp(p[0], e));
-        im[j] = d.imagSign * rint(ldexp(p[1], e));
+        vr = rint(ldexp(p[0], e));
+        vi = d.imagSign * rint(ldexp(p[1], e));
       } else {
-        re[j] = rint(ldexp(d.X[l * d.ld + row], e));
+        vr = rint(ldexp(d.X[l * d.ld + row], e));
       }
     }
+    re[j] = pieces16(vr);
+    im[j] = pieces16(vi);
   }
   const int wr = threadIdx.x >> 2;
   const int ws = threadIdx.x & 3;
   for (int t = 0; t < d.ms.count; ++t) {
-    const int mi = d.ms.m[t];
-    const double m = mi, inv = d.ms.inv[t];
+    const int mi = d.ms.m[t], c16 = d.ms.c16[t], c32 = d.ms.c32[t], c48 = d.ms.c48[t];
+    const float invf = d.ms.invf[t];
     const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const int r0 = sym_mod(re[j], m, inv, lo, hi);
+      const int r0 = sym_mod4(re[j], mi, c16, c32, c48, invf, lo, hi);
       lds[0][tx][ty + 4 * j] = static_cast<int8_t>(r0);
       if (CPLX) {
-        const int r1 = sym_mod(im[j], m, inv, lo, hi);
+        const int r1 = sym_mod4(im[j], mi, c16, c32, c48, invf, lo, hi);
         lds[1 % P][tx][ty + 4 * j] = static_cast<int8_t>(r1);
         lds[2 % P][tx][ty + 4 * j] = static_cast<int8_t>(sym_fold(r0 + r1, mi, lo, hi));
       }
@@ -290,10 +296,11 @@ __device__ __forceinline__ bool tile_active(int mode, int64_t rowOff, int64_t co
 // height (256) so that every load of a tile is in bounds without masks; padded
 // rows only feed output rows / columns that are never stored.
 //
-// Pipeline per k-block (BK = 128 bytes): global loads run two k-blocks ahead
-// through two register sets, LDS is double buffered (one barrier per k-block)
-// and the MFMA operands of the next 32-byte step are read from LDS while the
-// current step's 16 MFMAs issue.
+// Pipeline per k-block (BK = 128 bytes): global loads run about two k-blocks
+// ahead through two register sets, LDS is double buffered (one barrier per
+// k-block), and every memory instruction of a step (global loads, LDS writes of
+// the next k-block, LDS operand reads of the next 32-byte step) is interleaved
+// with its 64 MFMAs.
 template <bool OUT8, int WM, int WN>
 __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   constexpr int BM = 2 * WM;
@@ -380,7 +387,20 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
     for (int u = 0; u < CB; ++u)
       *reinterpret_cast<int4v*>(&lB[buf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = rb[u];
   };
-  auto compute = [&](int buf) {
+  // One k-block step over LDS buffer buf. Its 64 MFMAs (4 steps of 32 bytes x
+  // 16) are issued in groups of 4; between the groups go, spread evenly: the
+  // 16 global loads of k-block kb into register set (la, lb), the LDS operand
+  // reads of the next 32-byte step, and (during the last two steps) the 16 LDS
+  // writes of register set (sa, sb) into buffer sbuf. Spreading the memory
+  // instructions keeps their queues short, so they never hold up MFMA issue.
+  auto step = [&](int buf, int4v (&la)[CA], int4v (&lb)[CB], int kb, const int4v (&sa)[CA],
+                  const int4v (&sb)[CB], int sbuf) {
+    // groups of FN MFMAs: FM per 32-byte step, 4 FM per k-block; one global load per
+    // group, two LDS writes per group of the last two steps
+    static_assert(FM == FN && CA == CB && CA + CB == 4 * FM && BK == 128, "tile schedule");
+    const int64_t k0 = kBegin + static_cast<int64_t>(min(kb, nk - 1)) * BK;
+    const int8_t* Ak = At + k0;
+    const int8_t* Bk = Bt + k0;
     int4v a[2][FM], b[2][FN];
 #pragma unroll
     for (int f = 0; f < FM; ++f)
@@ -388,27 +408,48 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
 #pragma unroll
     for (int q = 0; q < FN; ++q)
       b[0][q] = *reinterpret_cast<const int4v*>(&lB[buf][wn * WN + 32 * q + l32][16 * khalf]);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ks = 0; ks < BK / 32; ++ks) {
       const int cur = ks & 1;
-      if (ks + 1 < BK / 32) {
 #pragma unroll
-        for (int f = 0; f < FM; ++f)
-          a[cur ^ 1][f] = *reinterpret_cast<const int4v*>(
-              &lA[buf][wm * WM + 32 * f + l32][(ks + 1) * 32 + 16 * khalf]);
+      for (int h = 0; h < FM; ++h) {
 #pragma unroll
         for (int q = 0; q < FN; ++q)
-          b[cur ^ 1][q] = *reinterpret_cast<const int4v*>(
-              &lB[buf][wn * WN + 32 * q + l32][(ks + 1) * 32 + 16 * khalf]);
+          acc[h][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cur][h], b[cur][q], acc[h][q], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        const int gi = FM * ks + h;  // global load of this group
+        if (gi < CA)
+          la[gi] = *reinterpret_cast<const int4v*>(Ak + voff[gi]);
+        else
+          lb[gi - CA] = *reinterpret_cast<const int4v*>(Bk + voff[gi - CA]);
+        if (ks + 1 < BK / 32) {
+          // next step's operands, B fragments first: the first group of the next
+          // step needs all of B but only the first A fragment
+#pragma unroll
+          for (int e = 2 * h; e < 2 * h + 2; ++e) {
+            if (e < FN)
+              b[cur ^ 1][e] = *reinterpret_cast<const int4v*>(
+                  &lB[buf][wn * WN + 32 * e + l32][(ks + 1) * 32 + 16 * khalf]);
+            else
+              a[cur ^ 1][e - FN] = *reinterpret_cast<const int4v*>(
+                  &lA[buf][wm * WM + 32 * (e - FN) + l32][(ks + 1) * 32 + 16 * khalf]);
+          }
+        }
+        if (ks >= BK / 32 - 2) {
+          const int wi = 2 * (FM * (ks - (BK / 32 - 2)) + h);  // 0, 2, .., CA + CB - 2
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            const int u = wi + e;
+            if (u < CA)
+              *reinterpret_cast<int4v*>(&lA[sbuf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = sa[u];
+            else
+              *reinterpret_cast<int4v*>(&lB[sbuf][(tid >> 3) + 32 * (u - CA)][16 * (tid & 7)]) =
+                  sb[u - CA];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      // the next step's LDS reads stay ahead of this step's MFMAs
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int f = 0; f < FM; ++f)
-#pragma unroll
-        for (int q = 0; q < FN; ++q)
-          acc[f][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cur][f], b[cur][q], acc[f][q], 0, 0, 0);
-      __builtin_amdgcn_sched_barrier(0);
     }
   };
 
@@ -421,17 +462,11 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
     __syncthreads();
     for (int it = 0; it < nk; it += 2) {
       // even step: LDS buffer 0 holds k-block it, register set 1 holds it + 1;
-      // the scheduling barriers keep the global loads ahead of the MFMAs
-      load(ra0, rb0, it + 2);
-      __builtin_amdgcn_sched_barrier(0);
-      compute(0);
-      store(ra1, rb1, 1);
+      // set 0 receives k-block it + 2
+      step(0, ra0, rb0, it + 2, ra1, rb1, 1);
       __syncthreads();
       // odd step: LDS buffer 1 holds k-block it + 1, register set 0 holds it + 2
-      load(ra1, rb1, it + 3);
-      __builtin_amdgcn_sched_barrier(0);
-      compute(1);
-      store(ra0, rb0, 0);
+      step(1, ra1, rb1, it + 3, ra0, rb0, 0);
       __syncthreads();
     }
   }
@@ -641,7 +676,7 @@ hipError_t launch_split_stats(bool complex, int layout, const double* X, int64_t
 hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipStream_t stream) {
   if (d.rows == 0) return hipSuccess;
   if (layout == KMajor) {
-    const dim3 grid(static_cast<unsigned>((d.kpad / 16 + 255) / 256), d.rows);
+    const dim3 grid(static_cast<unsigned>((d.kpad + kResElems - 1) / kResElems), d.rows);
     if (complex)
       hipLaunchKernelGGL(residue_kmajor_kernel<true>, grid, dim3(256), 0, stream, d);
     else

@@ -23,6 +23,11 @@ struct ModuliSet {
   int count;
   int m[kMaxModuli];
   double inv[kMaxModuli];  // 1.0 / m
+  // integer residues: 2^16, 2^32, 2^48 mod m and 1 / m rounded to float
+  int c16[kMaxModuli];
+  int c32[kMaxModuli];
+  int c48[kMaxModuli];
+  float invf[kMaxModuli];
 };
 
 // CRT constants of one moduli set (little-endian 64-bit limbs).
@@ -39,7 +44,7 @@ struct ResidueDesc {
   int64_t ld;       // leading dimension in elements
   int rows;
   int64_t k;
-  int64_t kpad;     // padded reduction length of the residue planes (multiple of 128)
+  int64_t kpad;     // padded reduction length of the residue planes (multiple of 256)
   const int* expo;  // per-row scaling exponent
   int imagSign;     // -1: the operand is read conjugated (complex only)
   int8_t* R;        // planes [count * products][rows][kpad]

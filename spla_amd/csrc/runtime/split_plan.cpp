@@ -134,8 +134,13 @@ void make_moduli(int count, ModuliSet& ms) {
   std::memset(&ms, 0, sizeof(ms));
   ms.count = count;
   for (int t = 0; t < count; ++t) {
-    ms.m[t] = kModuliList[t];
-    ms.inv[t] = 1.0 / kModuliList[t];
+    const int m = kModuliList[t];
+    ms.m[t] = m;
+    ms.inv[t] = 1.0 / m;
+    ms.c16[t] = static_cast<int>((int64_t(1) << 16) % m);
+    ms.c32[t] = static_cast<int>((int64_t(1) << 32) % m);
+    ms.c48[t] = static_cast<int>((int64_t(1) << 48) % m);
+    ms.invf[t] = 1.0f / static_cast<float>(m);
   }
 }
 
