@@ -28,10 +28,11 @@
 namespace spla_amd {
 namespace {
 
+using residue::ModConst;
 using residue::Pieces;
-using residue::pieces16;
+using residue::pieces14;
 using residue::sym_fold;
-using residue::sym_mod4;
+using residue::sym_mod;
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 typedef int int16v __attribute__((ext_vector_type(16)));
@@ -191,25 +192,26 @@ __global__ void __launch_bounds__(256) residue_kmajor_kernel(ResidueDesc d) {
   for (int j = 0; j < 4; ++j) {
     if (CPLX) {
       const double2 v = lds[res_slot(4 * tid + j)];
-      re[j] = pieces16(rint(ldexp(v.x, e)));
-      im[j] = pieces16(d.imagSign * rint(ldexp(v.y, e)));
+      re[j] = pieces14(rint(ldexp(v.x, e)));
+      im[j] = pieces14(d.imagSign * rint(ldexp(v.y, e)));
     } else {
       const double2 v = lds[res_slot(2 * tid + (j >> 1))];
-      re[j] = pieces16(rint(ldexp((j & 1) ? v.y : v.x, e)));
-      im[j] = Pieces{0, 0, 0, 0};
+      re[j] = pieces14(rint(ldexp((j & 1) ? v.y : v.x, e)));
+      im[j] = Pieces{0.f, 0.f, 0.f, 0.f};
     }
   }
   constexpr int P = CPLX ? 3 : 1;
   int8_t* base = d.R + static_cast<int64_t>(row) * d.kpad + l0;
   for (int t = 0; t < d.ms.count; ++t) {
-    const int mi = d.ms.m[t], c16 = d.ms.c16[t], c32 = d.ms.c32[t], c48 = d.ms.c48[t];
-    const float invf = d.ms.invf[t];
+    const int mi = d.ms.m[t];
     const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
+    const ModConst kc{static_cast<float>(mi), d.ms.invf[t], d.ms.c14[t], d.ms.c28[t],
+                      d.ms.c42[t], hi};
     int r0[4], r1[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      r0[j] = sym_mod4(re[j], mi, c16, c32, c48, invf, lo, hi);
-      r1[j] = CPLX ? sym_mod4(im[j], mi, c16, c32, c48, invf, lo, hi) : 0;
+      r0[j] = sym_mod(re[j], kc);
+      r1[j] = CPLX ? sym_mod(im[j], kc) : 0;
     }
     int8_t* p = base + static_cast<int64_t>(t * P) * d.plane;
     *reinterpret_cast<uint32_t*>(p) = pack4(r0[0], r0[1], r0[2], r0[3]);
@@ -249,21 +251,22 @@ __global__ void __launch_bounds__(256) residue_mnmajor_kernel(ResidueDesc d) {
         vr = rint(ldexp(d.X[l * d.ld + row], e));
       }
     }
-    re[j] = pieces16(vr);
-    im[j] = pieces16(vi);
+    re[j] = pieces14(vr);
+    im[j] = pieces14(vi);
   }
   const int wr = threadIdx.x >> 2;
   const int ws = threadIdx.x & 3;
   for (int t = 0; t < d.ms.count; ++t) {
-    const int mi = d.ms.m[t], c16 = d.ms.c16[t], c32 = d.ms.c32[t], c48 = d.ms.c48[t];
-    const float invf = d.ms.invf[t];
+    const int mi = d.ms.m[t];
     const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
+    const ModConst kc{static_cast<float>(mi), d.ms.invf[t], d.ms.c14[t], d.ms.c28[t],
+                      d.ms.c42[t], hi};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const int r0 = sym_mod4(re[j], mi, c16, c32, c48, invf, lo, hi);
+      const int r0 = sym_mod(re[j], kc);
       lds[0][tx][ty + 4 * j] = static_cast<int8_t>(r0);
       if (CPLX) {
-        const int r1 = sym_mod4(im[j], mi, c16, c32, c48, invf, lo, hi);
+        const int r1 = sym_mod(im[j], kc);
         lds[1 % P][tx][ty + 4 * j] = static_cast<int8_t>(r1);
         lds[2 % P][tx][ty + 4 * j] = static_cast<int8_t>(sym_fold(r0 + r1, mi, lo, hi));
       }

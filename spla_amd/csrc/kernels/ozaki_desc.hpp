@@ -23,10 +23,11 @@ struct ModuliSet {
   int count;
   int m[kMaxModuli];
   double inv[kMaxModuli];  // 1.0 / m
-  // integer residues: 2^16, 2^32, 2^48 mod m and 1 / m rounded to float
-  int c16[kMaxModuli];
-  int c32[kMaxModuli];
-  int c48[kMaxModuli];
+  // residue arithmetic in FP32: symmetric residues of 2^14, 2^28, 2^42 mod m
+  // and 1 / m rounded to float
+  float c14[kMaxModuli];
+  float c28[kMaxModuli];
+  float c42[kMaxModuli];
   float invf[kMaxModuli];
 };
 

@@ -1,5 +1,7 @@
 #include "runtime/split_plan.hpp"
 
+#include "kernels/residue_arith.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -137,9 +139,9 @@ void make_moduli(int count, ModuliSet& ms) {
     const int m = kModuliList[t];
     ms.m[t] = m;
     ms.inv[t] = 1.0 / m;
-    ms.c16[t] = static_cast<int>((int64_t(1) << 16) % m);
-    ms.c32[t] = static_cast<int>((int64_t(1) << 32) % m);
-    ms.c48[t] = static_cast<int>((int64_t(1) << 48) % m);
+    ms.c14[t] = static_cast<float>(residue::sym_pow2(14, m));
+    ms.c28[t] = static_cast<float>(residue::sym_pow2(28, m));
+    ms.c42[t] = static_cast<float>(residue::sym_pow2(42, m));
     ms.invf[t] = 1.0f / static_cast<float>(m);
   }
 }
