@@ -44,6 +44,11 @@ ALPHA = 1.0 + 0.0j
 BETA = 0.5 + 0.0j
 FP64_PEAK_TFLOPS = 78.6
 INT8_PEAK_TOPS = 5000.0
+# HBM bytes (read + write) per launch of the dominant kernel of each form on this
+# workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes (bench.py
+# --copy-calibration ...), each scaled by what the counter reads for a 1 GiB device
+# copy (reads x 2.0, writes x 1.0). They change only when the kernels change.
+HBM_TRAFFIC_BYTES = {"split": 67.06e9 + 1.51e9, "plain": 244.86e9 + 0.07e9}
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
 
@@ -267,7 +272,8 @@ def main() -> int:
         achieved = ops / (gemm_ms / 1e3) / 1e12
         roofline = {"bound": "mfma", "kernel": "i8_gemm_kernel (v_mfma_i32_32x32x32_i8)",
                     "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                    "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None}
+                    "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                    "traffic": HBM_TRAFFIC_BYTES["split"]}
         dtype = "i8"
     else:
         gemm_ms = best["phase_ms_per_step"].get("gemm_fp64", best["device_ms_per_step"])
@@ -275,7 +281,7 @@ def main() -> int:
         roofline = {"bound": "mfma", "kernel": "gemm_f64_kernel (v_mfma_f64_16x16x4_f64)",
                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                    "traffic": None}
+                    "traffic": HBM_TRAFFIC_BYTES["plain"]}
         dtype = "f64"
     line = {
         "metric": METRIC,
