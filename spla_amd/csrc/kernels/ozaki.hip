@@ -500,8 +500,8 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
             }
           }
         } else {
-          int* dst = static_cast<int*>(d.out) +
-                     (static_cast<int64_t>(s) * d.G + g) * d.outPlane +
+          int* dst = static_cast<int*>(d.out) + static_cast<int64_t>(s) * d.outSplitStride +
+                     static_cast<int64_t>(g) * d.outPlane +
                      static_cast<int64_t>(col) * d.m + row;
 #pragma unroll
           for (int r = 0; r < 4; ++r)

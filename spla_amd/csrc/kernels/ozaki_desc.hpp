@@ -64,8 +64,9 @@ struct I8GemmDesc {
   int splits;
   int64_t kSplit;  // reduction bytes per split (multiple of 256)
   int tilesM, tilesN;
-  void* out;       // uint8 residues [G][n][m] (splits == 1) or int32 sums [splits][G][n][m]
+  void* out;       // uint8 residues [G][n][m] (splits == 1) or int32 sums [splits][G'][n][m]
   int64_t outPlane;  // m * n
+  int64_t outSplitStride;  // int32 sums: elements from one split to the next (G' m n)
   ModuliSet ms;
   int fillMode;
   int64_t fillRowOff, fillColOff;

@@ -200,6 +200,7 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   gd.tilesM = static_cast<int>((m + bm - 1) / bm);
   gd.tilesN = static_cast<int>((n + bn - 1) / bn);
   gd.outPlane = m * n;
+  gd.outSplitStride = static_cast<int64_t>(G) * m * n;
   gd.ms = ms;
   gd.fillMode = fill.mode;
   gd.fillRowOff = fill.rowOff;
