@@ -93,12 +93,53 @@ def uniform_device(seed: int, mid: int, count: int, complex_: bool, device, chun
 
 # ---------------------------------------------------------------------------
 
+def _two_prod(x, y):
+    """Error-free product x * y = p + e (Veltkamp splitting, no FMA needed)."""
+    import numpy as np
+
+    def split(v):
+        c = v * 134217729.0  # 2^27 + 1
+        hi = c - (c - v)
+        return hi, v - hi
+
+    p = x * y
+    xh, xl = split(x)
+    yh, yl = split(y)
+    e = ((xh * yh - p) + xh * yl + xl * yh) + xl * yl
+    return p, e
+
+
+def _accurate_sum(v):
+    """Sum of v (last axis) with error-free pairwise TwoSum steps: the rounding
+    errors of every addition are kept and added at the end."""
+    import numpy as np
+
+    v = np.asarray(v, dtype=np.float64)
+    errs = np.zeros(v.shape[:-1])
+    while v.shape[-1] > 1:
+        if v.shape[-1] % 2:
+            v = np.concatenate([v, np.zeros(v.shape[:-1] + (1,))], axis=-1)
+        a, b = v[..., 0::2], v[..., 1::2]
+        s = a + b
+        bb = s - a
+        errs = errs + ((a - (s - bb)) + (b - bb)).sum(axis=-1)
+        v = s
+    return v[..., 0] + errs
+
+
+def _accurate_dot(x, y):
+    """sum_l x[..., l] * y[..., l] to about double-double accuracy."""
+    import numpy as np
+
+    p, e = _two_prod(x, y)
+    return _accurate_sum(np.concatenate([p, e], axis=-1))
+
+
 def sampled_error(dA, dB, dC0, dC, k, m, n, samples=48, seed=3):
-    """Normwise error estimate from sampled entries against the double-double oracle."""
+    """Normwise error estimate ||C - C_ref|| / (||A|| ||B||) from sampled entries;
+    C_ref = alpha A^H B + beta C0 with error-free products and compensated sums."""
     import numpy as np
     import torch
-
-    import oracle
 
     rng = np.random.default_rng(seed)
     rows = rng.integers(0, m, samples)
@@ -107,14 +148,15 @@ def sampled_error(dA, dB, dC0, dC, k, m, n, samples=48, seed=3):
     uc, cinv = np.unique(cols, return_inverse=True)
     A_cols = dA.view(m, k)[torch.from_numpy(ur).to(dA.device)].cpu().numpy()  # (len(ur), k)
     B_cols = dB.view(n, k)[torch.from_numpy(uc).to(dB.device)].cpu().numpy()
+    a = A_cols[rinv]
+    b = B_cols[cinv]
+    ar, ai, br, bi = a.real, a.imag, b.real, b.imag
+    # conj(a) . b = (ar br + ai bi) + i (ar bi - ai br)
+    re = _accurate_dot(np.concatenate([ar, ai], axis=-1), np.concatenate([br, bi], axis=-1))
+    im = _accurate_dot(np.concatenate([ar, -ai], axis=-1), np.concatenate([bi, br], axis=-1))
     C0 = dC0.view(n, m).cpu().numpy()  # column-major: C0[j, i]
     got = dC.view(n, m).cpu().numpy()
-    ref = np.empty(samples, dtype=np.complex128)
-    for s in range(samples):
-        a = np.ascontiguousarray(A_cols[rinv[s]])
-        b = np.ascontiguousarray(B_cols[cinv[s]])
-        c0 = np.array([C0[cols[s], rows[s]]], dtype=np.complex128)
-        ref[s] = oracle.gemm_entries(2, 0, k, ALPHA, a, k, b, k, BETA, c0, 1, [0], [0])[0]
+    ref = ALPHA * (re + 1j * im) + BETA * C0[cols, rows]
     diff = got[cols, rows] - ref
     est = np.sqrt(np.mean(np.abs(diff) ** 2) * m * n)
     nA = torch.linalg.vector_norm(dA).item()
@@ -160,6 +202,186 @@ def cpu_baseline(m, n, k_full, target_seconds=12.0):
     }
 
 
+WORKLOADS = ("ssb2048", "sbs100k", "ssb_real_spread", "batch512", "ssb2048_host")
+
+
+def _timed(fn, steps, warmup, stream):
+    """Runs fn warmup + steps times; returns (wall ms per step, event ms per step)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    return wall, e0.elapsed_time(e1) / steps
+
+
+def side_workload(args) -> int:
+    """The other parity workloads: sbs with a tall local A, a real ssb product with a
+    spread of exponents (and a range that forces the FP64 fall-back), 512 uneven
+    ring-step blocks through the batch API, and the headline product with
+    host-resident operands (PCIe-inclusive). One JSON line each."""
+    import numpy as np
+    import torch
+
+    import spla_amd as spla
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8}
+    steps, warmup = max(1, min(args.steps, 5)), max(1, min(args.warmup, 1))
+    out = {"metric": METRIC, "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "higher_is_better": True, "data": "synthetic (splitmix64 seed 42)", "forms": {}}
+    wl = args.workload
+
+    if wl == "sbs100k":
+        mL, n, k = 100000, 4096, 4096
+        dA = uniform_device(SEED, 1, mL * k, True, dev)
+        dB = uniform_device(SEED, 2, k * n, True, dev)
+        dC = torch.zeros(mL * n, dtype=torch.complex128, device=dev)
+        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
+                                                                 128, 128)
+        flops = 8.0 * mL * n * k
+        rows = np.random.default_rng(3).integers(0, mL, 32)
+        cols = np.random.default_rng(4).integers(0, n, 32)
+        a = dA.view(k, mL)[:, torch.from_numpy(rows).to(dev)].T.cpu().numpy()  # rows of A
+        b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy()
+        re = _accurate_dot(np.concatenate([a.real, -a.imag], -1), np.concatenate([b.real, b.imag], -1))
+        im = _accurate_dot(np.concatenate([a.real, a.imag], -1), np.concatenate([b.imag, b.real], -1))
+        ref = re + 1j * im
+        nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
+        for form in args.forms.split(","):
+            ctx.set_gemm_mode(modes[form])
+            run = lambda: spla.pgemm_sbs(mL, n, k, 1.0, dA, mL, dB, k, 0, 0, dist, 0.0, dC, mL, ctx)
+            wall, ev = _timed(run, steps, warmup, stream)
+            got = dC.view(n, mL).cpu().numpy()[cols, rows]
+            err = float(np.sqrt(np.mean(np.abs((got - ref) / nAB) ** 2) * mL * n))
+            info = ctx.last_product_info()
+            out["forms"][form] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
+                                  "device_ms_per_step": round(ev, 3), "normwise_error": err,
+                                  "split_used": info["split"], "moduli": info["moduli"]}
+            log(f"sbs100k {form}: {out['forms'][form]}")
+        out["config"] = {"workload": "spla_pzgemm_sbs complex, mLocal=100000 n=4096 k=4096, "
+                                     "device-resident, beta=0", "dist": "block cyclic 128x128 on 1x1"}
+        out["dtype"] = "c128"
+
+    elif wl == "ssb_real_spread":
+        m = n = 2048
+        k = 200000
+
+        def spread(mid, count, lo, hi):
+            x = uniform_device(SEED, mid, count, False, dev)
+            e = (uniform_device(SEED, mid + 100, count, False, dev) + 1.0) * 0.5 * (hi - lo + 1)
+            e = torch.clamp(torch.floor(e) + lo, lo, hi).to(torch.int32)
+            return torch.ldexp(x, e)
+
+        flops = 2.0 * m * n * k
+        rows = np.random.default_rng(3).integers(0, m, 32)
+        cols = np.random.default_rng(4).integers(0, n, 32)
+        for case, (lo, hi) in (("e in [-10, 10]", (-10, 10)), ("e in [-300, 300]", (-300, 300))):
+            dA = spread(11, k * m, lo, hi)
+            dB = spread(12, k * n, lo, hi)
+            dC = torch.zeros(m * n, dtype=torch.float64, device=dev)
+            dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
+                                                                     256, 256)
+            a = dA.view(m, k)[torch.from_numpy(rows).to(dev)].cpu().numpy()
+            b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy()
+            ref = _accurate_dot(a, b)
+            nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
+            for form in args.forms.split(","):
+                ctx.set_gemm_mode(modes[form])
+                run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.TRANSPOSE, 1.0, dA, k, dB, k,
+                                             0.0, dC, m, 0, 0, dist, ctx)
+                wall, ev = _timed(run, steps, warmup, stream)
+                got = dC.view(n, m).cpu().numpy()[cols, rows]
+                err = float(np.sqrt(np.mean(((got - ref) / nAB) ** 2) * m * n))
+                info = ctx.last_product_info()
+                out["forms"][f"{form}, {case}"] = {
+                    "tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
+                    "device_ms_per_step": round(ev, 3), "normwise_error": err,
+                    "split_used": info["split"], "fallback_reason": info["reason"],
+                    "moduli": info["moduli"], "bits": [info["bits_a"], info["bits_b"]]}
+                log(f"ssb_real_spread {form} {case}: {out['forms'][f'{form}, {case}']}")
+            del dA, dB, dC
+        out["config"] = {"workload": "spla_pdgemm_ssb A^T B real, m=n=2048 k=200000, entries "
+                                     "uniform[-1,1] * 2^e", "dist": "block cyclic 256x256 on 1x1"}
+        out["dtype"] = "f64"
+
+    elif wl == "batch512":
+        rng = np.random.default_rng(5)
+        cnt = 512
+        ms_ = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), cnt))).astype(int)
+        ns_ = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), cnt))).astype(int)
+        ks_ = np.rint(np.exp(rng.uniform(np.log(4096), np.log(65536), cnt))).astype(int)
+        offA = np.concatenate([[0], np.cumsum(ks_ * ms_)])
+        offB = np.concatenate([[0], np.cumsum(ks_ * ns_)])
+        offC = np.concatenate([[0], np.cumsum(ms_ * ns_)])
+        dA = uniform_device(SEED, 1, int(offA[-1]), True, dev)
+        dB = uniform_device(SEED, 2, int(offB[-1]), True, dev)
+        dC = torch.zeros(int(offC[-1]), dtype=torch.complex128, device=dev)
+        specs = []
+        full = tri = 0.0
+        for i in range(cnt):
+            mi, ni, ki = int(ms_[i]), int(ns_[i]), int(ks_[i])
+            fill = spla.FillMode.LOWER if i % 2 else spla.FillMode.FULL
+            specs.append(dict(m=mi, n=ni, k=ki, A=dA.data_ptr() + 16 * int(offA[i]), lda=ki,
+                              B=dB.data_ptr() + 16 * int(offB[i]), ldb=ki,
+                              C=dC.data_ptr() + 16 * int(offC[i]), ldc=mi, fill=fill))
+            full += 8.0 * mi * ni * ki
+            req = mi * ni if i % 2 == 0 else sum(min(ni, r + 1) for r in range(mi))
+            tri += 8.0 * req * ki
+        torch.cuda.synchronize()
+        for form in args.forms.split(","):
+            ctx.set_gemm_mode(modes[form])
+            batch = spla.Batch(ctx, specs, 1.0, 0.0)
+            wall, ev = _timed(batch.execute, steps, warmup, stream)
+            out["forms"][form] = {"tflops_full_blocks": round(full / wall / 1e9, 3),
+                                  "tflops_required_triangle": round(tri / wall / 1e9, 3),
+                                  "ms_per_step": round(wall, 3), "device_ms_per_step": round(ev, 3),
+                                  "plan": batch.info()}
+            batch.close()
+            log(f"batch512 {form}: {out['forms'][form]}")
+        out["config"] = {"workload": "512 complex ssb ring-step blocks A^H B through the batch API "
+                                     "(m, n log-uniform in [32, 512], k in [4096, 65536], seed 5; "
+                                     "odd blocks LOWER fill), alpha=1 beta=0",
+                         "flops_full_blocks": full, "flops_required_triangle": tri}
+        out["dtype"] = "c128"
+
+    elif wl == "ssb2048_host":
+        m, n, k = M, N, K
+        hA = uniform_device(SEED, 1, k * m, True, dev).cpu().pin_memory()
+        hB = uniform_device(SEED, 2, k * n, True, dev).cpu().pin_memory()
+        hC = uniform_device(SEED, 3, m * n, True, dev).cpu().pin_memory()
+        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
+                                                                 256, 256)
+        flops = 8.0 * m * n * k
+        for form in args.forms.split(","):
+            ctx.set_gemm_mode(modes[form])
+            run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, hA, k, hB, k,
+                                         BETA, hC, m, 0, 0, dist, ctx)
+            wall, ev = _timed(run, steps, warmup, stream)
+            info = ctx.last_product_info()
+            out["forms"][form] = {"tflops_pcie_inclusive": round(flops / wall / 1e9, 3),
+                                  "ms_per_step": round(wall, 3), "split_used": info["split"]}
+            log(f"ssb2048_host {form}: {out['forms'][form]}")
+        out["config"] = {"workload": "spla_pzgemm_ssb A^H B complex m=n=2048 k=200000, A, B, C in "
+                                     "pinned host memory (PCIe-inclusive; not the reported value)"}
+        out["dtype"] = "c128"
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,12 +392,17 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (reference kernel for HBM counters)")
+    ap.add_argument("--workload", default="ssb2048", choices=WORKLOADS,
+                    help="ssb2048 (default, the reported line) or one of the other parity "
+                         "workloads, each printed as its own JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU (single rank); --gpus must be 1",
               file=sys.stderr)
         return 2
+    if args.workload != "ssb2048":
+        return side_workload(args)
 
     import torch
 

@@ -325,27 +325,65 @@ void BatchPlan::execute() {
   const hipStream_t s = ctx_->compute_stream();
   ctx_->timer().reset();
   if (!kPersistentLaunch || ctx_->gemm_mode() != GemmModeFP64) {
-    // Block by block on the single-product engine. In split precision every
-    // block is its own split product (one host decision per block); blocks
-    // the split plan rejects run on the FP64 kernels.
+    // Block by block on the single-product engine, all enqueued on one stream
+    // and synchronized once (FP64), or one split product at a time (split
+    // precision: every block is its own plan with its own host decision; the
+    // residue planes of a block are released before the next one).
+    const bool split = ctx_->gemm_mode() != GemmModeFP64;
+    // FP64: independent blocks go round robin to the compute stream and the
+    // context's batch streams, so that small products run side by side
+    constexpr int kLanes = 1 + spla::ContextInternal::kBatchStreams;
+    hipStream_t lanes[kLanes] = {s};
+    int nLanes = 1;
+    if (!split) {
+      hipEvent_t start;
+      hip_check(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+      hip_check(hipEventRecord(start, s));
+      for (int i = 0; i < spla::ContextInternal::kBatchStreams; ++i) {
+        lanes[nLanes] = ctx_->batch_stream(i);
+        hip_check(hipStreamWaitEvent(lanes[nLanes++], start, 0));
+      }
+      (void)hipEventDestroy(start);
+    }
+    std::vector<Block> keep;
+    Block reuse[kLanes];
+    int next = 0;
     for (const BlockSpec& b : specs_) {
       if (b.m == 0 || b.n == 0) continue;
+      const int lane = next;
+      next = (next + 1) % nLanes;
+      const hipStream_t ls = lanes[lane];
       FillSpec fill;
       fill.mode = b.fillMode;
       fill.rowOff = b.rowOffset;
       fill.colOff = b.colOffset;
       if (complex_) {
         using Z = std::complex<double>;
-        gemm_gpu<Z>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
-                    static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B), b.ldb,
-                    Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_, fill);
+        enqueue_on_device<Z>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
+                             static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B), b.ldb,
+                             Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_, fill, ls,
+                             keep, &reuse[lane]);
       } else {
-        gemm_gpu<double>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
-                         static_cast<const double*>(b.A), b.lda,
-                         static_cast<const double*>(b.B), b.ldb, beta_[0],
-                         static_cast<double*>(b.C), b.ldc, *ctx_, fill);
+        enqueue_on_device<double>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
+                                  static_cast<const double*>(b.A), b.lda,
+                                  static_cast<const double*>(b.B), b.ldb, beta_[0],
+                                  static_cast<double*>(b.C), b.ldc, *ctx_, fill, ls, keep,
+                                  &reuse[lane]);
+      }
+      if (split) {
+        hip_check(hipStreamSynchronize(s));
+        keep.clear();
       }
     }
+    for (int i = 1; i < nLanes; ++i) {  // the compute stream joins the batch streams
+      hipEvent_t done;
+      hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+      hip_check(hipEventRecord(done, lanes[i]));
+      hip_check(hipStreamWaitEvent(s, done, 0));
+      (void)hipEventDestroy(done);
+    }
+    hip_check(hipStreamSynchronize(s));
+    if (ctx_->timer().enabled()) ctx_->timer().resolve();
     return;
   }
   if (exec_) {

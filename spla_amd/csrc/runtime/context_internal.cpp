@@ -44,6 +44,8 @@ ContextInternal::~ContextInternal() {
   // Pools release their blocks in their destructors; the streams go first.
   if (compute_) (void)hipStreamDestroy(compute_);
   if (copy_) (void)hipStreamDestroy(copy_);
+  for (hipStream_t b : batch_)
+    if (b) (void)hipStreamDestroy(b);
 }
 
 void ContextInternal::set_num_tiles(int v) {
@@ -92,6 +94,13 @@ hipStream_t ContextInternal::compute_stream() {
 hipStream_t ContextInternal::copy_stream() {
   ensure_device_resources();
   return copy_;
+}
+
+hipStream_t ContextInternal::batch_stream(int i) {
+  ensure_device_resources();
+  if (!batch_[i])  // default priority, like every stream of the engine
+    spla_amd::hip_check(hipStreamCreateWithFlags(&batch_[i], hipStreamNonBlocking));
+  return batch_[i];
 }
 
 int ContextInternal::num_cus() {

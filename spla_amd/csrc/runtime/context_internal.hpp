@@ -62,6 +62,9 @@ public:
   std::size_t stream_chunk_bytes() const { return streamChunkBytes_; }
   void set_stream_chunk_bytes(std::size_t b) { streamChunkBytes_ = b; }
   hipStream_t copy_stream();
+  // Extra streams for independent products of one batch (created on first use).
+  hipStream_t batch_stream(int i);
+  static constexpr int kBatchStreams = 3;
   int num_cus();
 
   spla_amd::Tuner& tuner() { return tuner_; }
@@ -95,6 +98,7 @@ private:
   hipStream_t userStream_ = nullptr;
   std::size_t streamChunkBytes_ = std::size_t(256) << 20;
   hipStream_t copy_ = nullptr;
+  hipStream_t batch_[kBatchStreams] = {nullptr, nullptr, nullptr};
   int numCUs_ = 0;
 
   spla_amd::Tuner tuner_;

@@ -457,6 +457,25 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
   }
 }
 
+template <typename T>
+void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
+                       T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C,
+                       int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
+                       hipStream_t stream, std::vector<Block>& keep, Block* reuse) {
+  enqueue_device_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
+                            stream, keep, reuse);
+}
+
+template void enqueue_on_device<double>(SplaOperation, SplaOperation, int64_t, int64_t, int64_t,
+                                        double, const double*, int64_t, const double*, int64_t,
+                                        double, double*, int64_t, spla::ContextInternal&,
+                                        const FillSpec&, hipStream_t, std::vector<Block>&, Block*);
+template void enqueue_on_device<std::complex<double>>(
+    SplaOperation, SplaOperation, int64_t, int64_t, int64_t, std::complex<double>,
+    const std::complex<double>*, int64_t, const std::complex<double>*, int64_t,
+    std::complex<double>, std::complex<double>*, int64_t, spla::ContextInternal&, const FillSpec&,
+    hipStream_t, std::vector<Block>&, Block*);
+
 template void gemm_gpu<float>(SplaOperation, SplaOperation, int64_t, int64_t, int64_t, float,
                               const float*, int64_t, const float*, int64_t, float, float*, int64_t,
                               spla::ContextInternal&, FillSpec);

@@ -19,7 +19,9 @@
 
 #include <complex>
 #include <cstdint>
+#include <vector>
 
+#include "runtime/allocators.hpp"
 #include "runtime/context_internal.hpp"
 #include "spla/types.h"
 
@@ -37,6 +39,16 @@ template <typename T>
 void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k, T alpha,
               const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
               spla::ContextInternal& ctx, FillSpec fill = FillSpec());
+
+// One product on device-resident operands, enqueued on `stream` with no
+// synchronization (T: double or std::complex<double>). Workspace blocks are
+// appended to `keep` and must live until the stream has drained; `reuse`
+// (optional) is one split-K workspace shared by products on the same stream.
+template <typename T>
+void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
+                       T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C,
+                       int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
+                       hipStream_t stream, std::vector<Block>& keep, Block* reuse);
 
 // SPLA's check_gemm_param (src/util/check_gemm_param.hpp:36-50): throws
 // InvalidParameterError / InvalidPointerError.
