@@ -28,11 +28,11 @@
 namespace spla_amd {
 namespace {
 
+using residue::mix;
 using residue::ModConst;
 using residue::Pieces;
 using residue::pieces14;
-using residue::sym_fold;
-using residue::sym_mod;
+using residue::residue_biased;
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 typedef int int16v __attribute__((ext_vector_type(16)));
@@ -147,9 +147,24 @@ __global__ void __launch_bounds__(256) stats_finalize_kernel(const double* pmax,
 // ---------------------------------------------------------------------------
 // residues
 
-__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
-  return (static_cast<uint32_t>(a) & 0xffu) | ((static_cast<uint32_t>(b) & 0xffu) << 8) |
-         ((static_cast<uint32_t>(c) & 0xffu) << 16) | ((static_cast<uint32_t>(d) & 0xffu) << 24);
+// Biased residue rb = r + 128 in [0, 255] (exact float) into byte j of w.
+__device__ __forceinline__ uint32_t put_byte(float rb, int j, uint32_t w) {
+  return __builtin_amdgcn_cvt_pk_u8_f32(rb, j, w);
+}
+
+// Four biased bytes -> four int8 residues (r + 128 with the top bit flipped is r).
+constexpr uint32_t kByteSigns = 0x80808080u;
+
+// The three planes of one element and modulus: Re (from the biased pieces,
+// p0 + 128), Im (unbiased pieces) and (Re + Im) mod m, each as a biased byte.
+struct Biased3 {
+  float re, im, sum;
+};
+__device__ __forceinline__ Biased3 residues3(const Pieces& reb, const Pieces& im,
+                                             const ModConst& kc) {
+  const float rre = residue_biased(mix(reb, reb.p0, kc), kc);
+  const float rim = residue::residue(mix(im, im.p0, kc), kc);
+  return {rre, rim + 128.0f, residue_biased(rre + rim, kc)};
 }
 
 // Reduction index contiguous: a block converts 1024 consecutive elements of a
@@ -187,7 +202,7 @@ __global__ void __launch_bounds__(256) residue_kmajor_kernel(ResidueDesc d) {
   const int64_t l0 = e0 + 4 * tid;  // first of this thread's 4 elements
   if (l0 >= d.kpad) return;
   const int e = d.expo[row];
-  Pieces re[4], im[4];
+  Pieces re[4], im[4];  // re carries the +128 of the biased form in p0
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (CPLX) {
@@ -199,48 +214,54 @@ __global__ void __launch_bounds__(256) residue_kmajor_kernel(ResidueDesc d) {
       re[j] = pieces14(rint(ldexp((j & 1) ? v.y : v.x, e)));
       im[j] = Pieces{0.f, 0.f, 0.f, 0.f};
     }
+    re[j].p0 += 128.0f;
   }
   constexpr int P = CPLX ? 3 : 1;
   int8_t* base = d.R + static_cast<int64_t>(row) * d.kpad + l0;
   for (int t = 0; t < d.ms.count; ++t) {
-    const int mi = d.ms.m[t];
-    const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
-    const ModConst kc{static_cast<float>(mi), d.ms.invf[t], d.ms.c14[t], d.ms.c28[t],
-                      d.ms.c42[t], hi};
-    int r0[4], r1[4];
+    const ModConst kc = d.ms.kc[t];
+    uint32_t w0 = 0, w1 = 0, w2 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      r0[j] = sym_mod(re[j], kc);
-      r1[j] = CPLX ? sym_mod(im[j], kc) : 0;
+      if (CPLX) {
+        const Biased3 r = residues3(re[j], im[j], kc);
+        w0 = put_byte(r.re, j, w0);
+        w1 = put_byte(r.im, j, w1);
+        w2 = put_byte(r.sum, j, w2);
+      } else {
+        w0 = put_byte(residue_biased(mix(re[j], re[j].p0, kc), kc), j, w0);
+      }
     }
     int8_t* p = base + static_cast<int64_t>(t * P) * d.plane;
-    *reinterpret_cast<uint32_t*>(p) = pack4(r0[0], r0[1], r0[2], r0[3]);
+    *reinterpret_cast<uint32_t*>(p) = w0 ^ kByteSigns;
     if (CPLX) {
-      *reinterpret_cast<uint32_t*>(p + d.plane) = pack4(r1[0], r1[1], r1[2], r1[3]);
-      *reinterpret_cast<uint32_t*>(p + 2 * d.plane) =
-          pack4(sym_fold(r0[0] + r1[0], mi, lo, hi), sym_fold(r0[1] + r1[1], mi, lo, hi),
-                sym_fold(r0[2] + r1[2], mi, lo, hi), sym_fold(r0[3] + r1[3], mi, lo, hi));
+      *reinterpret_cast<uint32_t*>(p + d.plane) = w1 ^ kByteSigns;
+      *reinterpret_cast<uint32_t*>(p + 2 * d.plane) = w2 ^ kByteSigns;
     }
   }
 }
 
-// Rows contiguous: a 64 x 64 tile is read along rows and transposed through LDS
-// so that the planes are written along the reduction index.
+// Rows contiguous: a 64-row x 128-element tile is read along rows (a wave
+// reads 64 consecutive rows of one reduction index), every thread converts 16
+// consecutive reduction indices of its row into one 16-byte LDS word per
+// plane, and the tile leaves along the reduction index (128 contiguous bytes
+// per row and plane).
+constexpr int kResMnK = 128;  // reduction indices per block
 template <bool CPLX>
-__global__ void __launch_bounds__(256) residue_mnmajor_kernel(ResidueDesc d) {
+__global__ void __launch_bounds__(512) residue_mnmajor_kernel(ResidueDesc d) {
   constexpr int P = CPLX ? 3 : 1;
-  constexpr int kStride = 64 + 16;
+  constexpr int kStride = kResMnK + 16;
   __shared__ __attribute__((aligned(16))) int8_t lds[P][64][kStride];
   const int tx = threadIdx.x & 63;
-  const int ty = threadIdx.x >> 6;
+  const int ty = threadIdx.x >> 6;  // 0..7: reduction indices 16 ty .. 16 ty + 15
   const int row0 = blockIdx.y * 64;
-  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64;
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kResMnK + 16 * ty;
   const int row = row0 + tx;
   const int e = row < d.rows ? d.expo[row] : 0;
-  Pieces re[16], im[16];
+  Pieces re[16], im[16];  // re carries the +128 of the biased form in p0
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    const int64_t l = k0 + ty + 4 * j;
+    const int64_t l = k0 + j;
     double vr = 0.0, vi = 0.0;
     if (row < d.rows && l < d.k) {
       if (CPLX) {
@@ -252,31 +273,44 @@ __global__ void __launch_bounds__(256) residue_mnmajor_kernel(ResidueDesc d) {
       }
     }
     re[j] = pieces14(vr);
+    re[j].p0 += 128.0f;
     im[j] = pieces14(vi);
   }
-  const int wr = threadIdx.x >> 2;
-  const int ws = threadIdx.x & 3;
+  const int wr = threadIdx.x >> 3;  // read-out: row wr, bytes 16 ws .. 16 ws + 15
+  const int ws = threadIdx.x & 7;
   for (int t = 0; t < d.ms.count; ++t) {
-    const int mi = d.ms.m[t];
-    const int lo = -(mi / 2), hi = mi - 1 - mi / 2;
-    const ModConst kc{static_cast<float>(mi), d.ms.invf[t], d.ms.c14[t], d.ms.c28[t],
-                      d.ms.c42[t], hi};
+    const ModConst kc = d.ms.kc[t];
+    int4v w[P];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int r0 = sym_mod(re[j], kc);
-      lds[0][tx][ty + 4 * j] = static_cast<int8_t>(r0);
+    for (int q = 0; q < 4; ++q) {
+      uint32_t w0 = 0, w1 = 0, w2 = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = 4 * q + b;
+        if (CPLX) {
+          const Biased3 r = residues3(re[j], im[j], kc);
+          w0 = put_byte(r.re, b, w0);
+          w1 = put_byte(r.im, b, w1);
+          w2 = put_byte(r.sum, b, w2);
+        } else {
+          w0 = put_byte(residue_biased(mix(re[j], re[j].p0, kc), kc), b, w0);
+        }
+      }
+      w[0][q] = static_cast<int>(w0 ^ kByteSigns);
       if (CPLX) {
-        const int r1 = sym_mod(im[j], kc);
-        lds[1 % P][tx][ty + 4 * j] = static_cast<int8_t>(r1);
-        lds[2 % P][tx][ty + 4 * j] = static_cast<int8_t>(sym_fold(r0 + r1, mi, lo, hi));
+        w[1 % P][q] = static_cast<int>(w1 ^ kByteSigns);
+        w[2 % P][q] = static_cast<int>(w2 ^ kByteSigns);
       }
     }
+#pragma unroll
+    for (int p = 0; p < P; ++p) *reinterpret_cast<int4v*>(&lds[p][tx][16 * ty]) = w[p];
     __syncthreads();
     if (row0 + wr < d.rows) {
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         int8_t* dst = d.R + static_cast<int64_t>(t * P + p) * d.plane +
-                      static_cast<int64_t>(row0 + wr) * d.kpad + k0 + 16 * ws;
+                      static_cast<int64_t>(row0 + wr) * d.kpad +
+                      static_cast<int64_t>(blockIdx.x) * kResMnK + 16 * ws;
         *reinterpret_cast<int4v*>(dst) = *reinterpret_cast<const int4v*>(&lds[p][wr][16 * ws]);
       }
     }
@@ -685,11 +719,11 @@ hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipSt
     else
       hipLaunchKernelGGL(residue_kmajor_kernel<false>, grid, dim3(256), 0, stream, d);
   } else {
-    const dim3 grid(static_cast<unsigned>(d.kpad / 64), (d.rows + 63) / 64);
+    const dim3 grid(static_cast<unsigned>(d.kpad / kResMnK), (d.rows + 63) / 64);
     if (complex)
-      hipLaunchKernelGGL(residue_mnmajor_kernel<true>, grid, dim3(256), 0, stream, d);
+      hipLaunchKernelGGL(residue_mnmajor_kernel<true>, grid, dim3(512), 0, stream, d);
     else
-      hipLaunchKernelGGL(residue_mnmajor_kernel<false>, grid, dim3(256), 0, stream, d);
+      hipLaunchKernelGGL(residue_mnmajor_kernel<false>, grid, dim3(512), 0, stream, d);
   }
   return hipGetLastError();
 }

@@ -15,6 +15,8 @@
 
 #include <cstdint>
 
+#include "kernels/residue_arith.hpp"
+
 namespace spla_amd {
 
 constexpr int kMaxModuli = 20;
@@ -23,12 +25,8 @@ struct ModuliSet {
   int count;
   int m[kMaxModuli];
   double inv[kMaxModuli];  // 1.0 / m
-  // residue arithmetic in FP32: symmetric residues of 2^14, 2^28, 2^42 mod m
-  // and 1 / m rounded to float
-  float c14[kMaxModuli];
-  float c28[kMaxModuli];
-  float c42[kMaxModuli];
-  float invf[kMaxModuli];
+  // constants of the FP32 residue arithmetic (residue_arith.hpp)
+  residue::ModConst kc[kMaxModuli];
 };
 
 // CRT constants of one moduli set (little-endian 64-bit limbs).

@@ -139,10 +139,7 @@ void make_moduli(int count, ModuliSet& ms) {
     const int m = kModuliList[t];
     ms.m[t] = m;
     ms.inv[t] = 1.0 / m;
-    ms.c14[t] = static_cast<float>(residue::sym_pow2(14, m));
-    ms.c28[t] = static_cast<float>(residue::sym_pow2(28, m));
-    ms.c42[t] = static_cast<float>(residue::sym_pow2(42, m));
-    ms.invf[t] = 1.0f / static_cast<float>(m);
+    ms.kc[t] = residue::mod_const(m);
   }
 }
 
