@@ -224,6 +224,18 @@ def _timed(fn, steps, warmup, stream):
     return wall, e0.elapsed_time(e1) / steps
 
 
+def _phases(ctx, fn) -> dict:
+    """Per-phase device times (ms) of one extra call of fn."""
+    import torch
+
+    ctx.set_timing(True)
+    fn()
+    torch.cuda.synchronize()
+    ph = ctx.last_timing()
+    ctx.set_timing(False)
+    return {k: round(v, 3) for k, v in ph.items() if v > 0}
+
+
 def side_workload(args) -> int:
     """The other parity workloads: sbs with a tall local A, a real ssb product with a
     spread of exponents (and a range that forces the FP64 fall-back), 512 uneven
@@ -269,7 +281,8 @@ def side_workload(args) -> int:
             info = ctx.last_product_info()
             out["forms"][form] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
                                   "device_ms_per_step": round(ev, 3), "normwise_error": err,
-                                  "split_used": info["split"], "moduli": info["moduli"]}
+                                  "split_used": info["split"], "moduli": info["moduli"],
+                                  "phase_ms": _phases(ctx, run)}
             log(f"sbs100k {form}: {out['forms'][form]}")
         out["config"] = {"workload": "spla_pzgemm_sbs complex, mLocal=100000 n=4096 k=4096, "
                                      "device-resident, beta=0", "dist": "block cyclic 128x128 on 1x1"}
@@ -310,7 +323,8 @@ def side_workload(args) -> int:
                     "tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
                     "device_ms_per_step": round(ev, 3), "normwise_error": err,
                     "split_used": info["split"], "fallback_reason": info["reason"],
-                    "moduli": info["moduli"], "bits": [info["bits_a"], info["bits_b"]]}
+                    "moduli": info["moduli"], "bits": [info["bits_a"], info["bits_b"]],
+                    "phase_ms": _phases(ctx, run)}
                 log(f"ssb_real_spread {form} {case}: {out['forms'][f'{form}, {case}']}")
             del dA, dB, dC
         out["config"] = {"workload": "spla_pdgemm_ssb A^T B real, m=n=2048 k=200000, entries "

@@ -37,7 +37,8 @@ hipError_t launch_split_stats(bool complex, int layout, const double* X, int64_t
                               double* outSum, hipStream_t stream);
 hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipStream_t stream);
 hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream);
-hipError_t launch_crt(bool complex, const CrtDesc& d, hipStream_t stream);
+// limbs: CrtConstants::limbs of the constants d.cc points to
+hipError_t launch_crt(bool complex, const CrtDesc& d, int limbs, hipStream_t stream);
 
 // Persistent batched block products (batched.hip); cfg is TileL or TileM.
 hipError_t launch_batched_ssb(const BatchDesc& d, bool complex, int cfg, int workgroups,

@@ -21,6 +21,7 @@
 
 #include <cstdint>
 
+#include "kernels/crt_arith.hpp"
 #include "kernels/launch.hpp"
 #include "kernels/ozaki_desc.hpp"
 #include "kernels/residue_arith.hpp"
@@ -509,9 +510,12 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   }
 
   // D layout of v_mfma_i32_32x32x32_i8: lane holds column lane % 32 and rows
-  // 8 * (r / 4) + 4 * (lane / 32) + r % 4, r = 0..15.
-  const int mod = d.ms.m[g / d.products];
-  const double dm = mod, dinv = d.ms.inv[g / d.products];
+  // 8 * (r / 4) + 4 * (lane / 32) + r % 4, r = 0..15: four consecutive rows per
+  // group of four registers, stored as one word (biased residues) or one
+  // 16-byte vector (int32 sums). Output columns hold ldo >= every tile row.
+  const int tmod = g / d.products;
+  const double dm = d.ms.m[tmod], dinv = d.ms.inv[tmod];
+  const double tie = d.ms.kc[tmod].tie;  // m = 256: ties go up, r in [-128, 127]
 #pragma unroll
   for (int f = 0; f < FM; ++f) {
 #pragma unroll
@@ -522,24 +526,25 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
       for (int rb4 = 0; rb4 < 4; ++rb4) {
         const int row = row0 + wm * WM + 32 * f + 8 * rb4 + 4 * khalf;
         if (OUT8) {
-          uint8_t* dst = static_cast<uint8_t*>(d.out) + static_cast<int64_t>(g) * d.outPlane +
-                         static_cast<int64_t>(col) * d.m + row;
+          uint32_t w = 0;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            if (row + r < d.m) {
-              const double v = acc[f][q][4 * rb4 + r];
-              double rr = v - dm * floor(v * dinv);
-              rr = rr < 0.0 ? rr + dm : (rr >= dm ? rr - dm : rr);
-              dst[r] = static_cast<uint8_t>(static_cast<int>(rr));
-            }
+            // symmetric residue of the INT32 sum: |v| < 2^31, q = round(v / m) exact
+            const double v = acc[f][q][4 * rb4 + r];
+            const double rr = fma(-rint(fma(v, dinv, tie)), dm, v);
+            w |= (static_cast<uint32_t>(static_cast<int>(rr) + 128) & 0xffu) << (8 * r);
           }
+          *reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(d.out) +
+                                       static_cast<int64_t>(g) * d.outPlane +
+                                       static_cast<int64_t>(col) * d.ldo + row) = w;
         } else {
-          int* dst = static_cast<int*>(d.out) + static_cast<int64_t>(s) * d.outSplitStride +
-                     static_cast<int64_t>(g) * d.outPlane +
-                     static_cast<int64_t>(col) * d.m + row;
+          int4v v;
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (row + r < d.m) dst[r] = acc[f][q][4 * rb4 + r];
+          for (int r = 0; r < 4; ++r) v[r] = acc[f][q][4 * rb4 + r];
+          *reinterpret_cast<int4v*>(static_cast<int*>(d.out) +
+                                    static_cast<int64_t>(s) * d.outSplitStride +
+                                    static_cast<int64_t>(g) * d.outPlane +
+                                    static_cast<int64_t>(col) * d.ldo + row) = v;
         }
       }
     }
@@ -549,117 +554,50 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
 // ---------------------------------------------------------------------------
 // Chinese remainder reconstruction and FP64 epilogue
 
-struct U192 {
-  uint64_t w0, w1, w2;
-};
-
-__device__ __forceinline__ void mac(U192& x, const uint64_t* W, uint64_t r) {
-  const uint64_t lo0 = W[0] * r, hi0 = __umul64hi(W[0], r);
-  const uint64_t lo1 = W[1] * r, hi1 = __umul64hi(W[1], r);
-  const uint64_t lo2 = W[2] * r;
-  x.w0 += lo0;
-  uint64_t c = x.w0 < lo0;
-  uint64_t t = lo1 + hi0;
-  uint64_t c1 = t < lo1;
-  t += c;
-  c1 += t < c;
-  x.w1 += t;
-  c1 += x.w1 < t;
-  x.w2 += lo2 + hi1 + c1;
-}
-
-__device__ __forceinline__ bool geq(const U192& a, const uint64_t* b) {
-  if (a.w2 != b[2]) return a.w2 > b[2];
-  if (a.w1 != b[1]) return a.w1 > b[1];
-  return a.w0 >= b[0];
-}
-
-__device__ __forceinline__ void sub_in(U192& a, const U192& b) {  // a -= b, a >= b
-  const uint64_t b0 = a.w0 < b.w0;
-  a.w0 -= b.w0;
-  const uint64_t t1 = b.w1 + b0;
-  const uint64_t b1 = (a.w1 < t1) | (t1 < b0);
-  a.w1 -= t1;
-  a.w2 -= b.w2 + b1;
-}
-
-__device__ __forceinline__ double to_double(const U192& x) {
-  return ldexp(static_cast<double>(x.w2), 128) + ldexp(static_cast<double>(x.w1), 64) +
-         static_cast<double>(x.w0);
-}
-
-// x mod M, then mapped to the symmetric range and converted to double.
-__device__ __forceinline__ double crt_finish(U192 x, const CrtConstants& cc) {
-  const double q = floor(to_double(x) / cc.Md);
-  U192 M{cc.M[0], cc.M[1], cc.M[2]};
-  if (q > 0.0) {
-    // subtract (q - 1) * M to stay non-negative, then fix up by comparison
-    const uint64_t qi = static_cast<uint64_t>(q) - 1;
-    U192 qm{0, 0, 0};
-    mac(qm, cc.M, qi);
-    if (geq(x, &qm.w0)) sub_in(x, qm);
+// Per output element and modulus: the symmetric residues of Re and Im of the
+// product (complex: Re = T1 - T2, Im = T3 - T1 - T2 modulo m), |r| <= m / 2.
+template <bool CPLX, bool IN8>
+__device__ __forceinline__ void product_residues(const CrtDesc& d, int t, int64_t off,
+                                                 double (&re)[4], double (&im)[4]) {
+  const double m = d.ms.m[t], inv = d.ms.inv[t];
+  double T[3][4];
+#pragma unroll
+  for (int p = 0; p < (CPLX ? 3 : 1); ++p) {
+    const int64_t g = static_cast<int64_t>(t) * d.products + p;
+    if (IN8) {
+      // biased residues r + 128 of four consecutive rows
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(
+          static_cast<const uint8_t*>(d.W) + g * d.plane + off);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) T[p][u] = static_cast<double>(static_cast<int>((w >> (8 * u)) & 0xffu) - 128);
+    } else {
+      const int* w = static_cast<const int*>(d.W);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) T[p][u] = 0.0;
+      for (int s = 0; s < d.splits; ++s) {
+        const int4v v = *reinterpret_cast<const int4v*>(
+            w + (static_cast<int64_t>(s) * d.G + g) * d.plane + off);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) T[p][u] += static_cast<double>(v[u]);  // exact: < 2^38
+      }
+    }
   }
-  for (int it = 0; it < 3 && geq(x, cc.M); ++it) sub_in(x, M);
-  if (geq(x, cc.halfM) && !(x.w0 == cc.halfM[0] && x.w1 == cc.halfM[1] && x.w2 == cc.halfM[2])) {
-    U192 neg = M;
-    sub_in(neg, x);
-    return -to_double(neg);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (CPLX) {
+      re[u] = crt::sym_residue(T[0][u] - T[1][u], m, inv);
+      im[u] = crt::sym_residue(T[2][u] - T[0][u] - T[1][u], m, inv);
+    } else {
+      re[u] = crt::sym_residue(T[0][u], m, inv);
+    }
   }
-  return to_double(x);
-}
-
-__device__ __forceinline__ uint64_t umod(int64_t v, int m, double inv) {
-  const double vd = static_cast<double>(v);  // |v| < 2^53
-  double r = vd - static_cast<double>(m) * floor(vd * inv);
-  r = r < 0.0 ? r + m : (r >= m ? r - m : r);
-  return static_cast<uint64_t>(r);
 }
 
 template <bool CPLX>
-__global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  const int64_t total = static_cast<int64_t>(d.m) * d.n;
-  if (idx >= total) return;
-  const int i = static_cast<int>(idx % d.m);
-  const int j = static_cast<int>(idx / d.m);
-  if (d.fillMode != 0) {
-    const int64_t r0 = (i / d.tileRows) * static_cast<int64_t>(d.tileRows);
-    const int64_t c0 = (j / d.tileCols) * static_cast<int64_t>(d.tileCols);
-    if (!tile_active(d.fillMode, d.fillRowOff, d.fillColOff, r0, c0,
-                     min<int64_t>(d.tileRows, d.m - r0), min<int64_t>(d.tileCols, d.n - c0)))
-      return;
-  }
-  const CrtConstants& cc = *d.cc;
-  U192 xr{0, 0, 0}, xi{0, 0, 0};
-  for (int t = 0; t < d.ms.count; ++t) {
-    const int m = d.ms.m[t];
-    const double inv = d.ms.inv[t];
-    int64_t T[3] = {0, 0, 0};
-#pragma unroll
-    for (int p = 0; p < (CPLX ? 3 : 1); ++p) {
-      const int g = t * d.products + p;
-      if (d.in8) {
-        T[p] = static_cast<const uint8_t*>(d.W)[static_cast<int64_t>(g) * d.plane + idx];
-      } else {
-        const int* w = static_cast<const int*>(d.W);
-        int64_t sum = 0;
-        for (int s = 0; s < d.splits; ++s)
-          sum += w[(static_cast<int64_t>(s) * d.G + g) * d.plane + idx];
-        T[p] = sum;
-      }
-    }
-    if (CPLX) {
-      mac(xr, cc.W[t], umod(T[0] - T[1], m, inv));
-      mac(xi, cc.W[t], umod(T[2] - T[0] - T[1], m, inv));
-    } else {
-      mac(xr, cc.W[t], umod(T[0], m, inv));
-    }
-  }
-  const int sc = -(d.expA[i] + d.expB[j]);
-  const double re = ldexp(crt_finish(xr, cc), sc);
-  const double im = CPLX ? ldexp(crt_finish(xi, cc), sc) : 0.0;
+__device__ __forceinline__ void update_c(const CrtDesc& d, int64_t i, int64_t j, double re,
+                                         double im) {
   if (CPLX) {
-    double* c = d.C + (static_cast<int64_t>(j) * d.ldc + i) * 2;
+    double* c = d.C + (j * d.ldc + i) * 2;
     double outRe = d.alphaRe * re - d.alphaIm * im;
     double outIm = d.alphaRe * im + d.alphaIm * re;
     if (!d.betaZero) {
@@ -670,10 +608,58 @@ __global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
     c[0] = outRe;
     c[1] = outIm;
   } else {
-    double* c = d.C + static_cast<int64_t>(j) * d.ldc + i;
+    double* c = d.C + j * d.ldc + i;
     double out = d.alphaRe * re;
     if (!d.betaZero) out += d.betaRe * c[0];
     c[0] = out;
+  }
+}
+
+// One thread per four consecutive rows of one column of C: limb sums of the
+// CRT over all moduli (crt_arith.hpp), canonical representative, scaling by
+// 2^-(ea_i + eb_j) and the alpha / beta update.
+template <bool CPLX, bool IN8, int L>
+__global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
+  const int64_t quads = (d.m + 3) / 4;
+  const int64_t qid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (qid >= quads * d.n) return;
+  const int64_t j = qid / quads;
+  const int64_t i0 = 4 * (qid - j * quads);
+  if (d.fillMode != 0) {
+    const int64_t r0 = (i0 / d.tileRows) * static_cast<int64_t>(d.tileRows);
+    const int64_t c0 = (j / d.tileCols) * static_cast<int64_t>(d.tileCols);
+    if (!tile_active(d.fillMode, d.fillRowOff, d.fillColOff, r0, c0,
+                     min<int64_t>(d.tileRows, d.m - r0), min<int64_t>(d.tileCols, d.n - c0)))
+      return;
+  }
+  const CrtConstants& cc = *d.cc;
+  const int64_t off = j * d.ldo + i0;
+  double Sre[4][L], Sim[4][L];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int l = 0; l < L; ++l) Sre[u][l] = Sim[u][l] = 0.0;
+  for (int t = 0; t < d.ms.count; ++t) {
+    double re[4], im[4];
+    product_residues<CPLX, IN8>(d, t, off, re, im);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const double w = cc.Wl[t][l];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        Sre[u][l] = fma(re[u], w, Sre[u][l]);  // exact (crt_arith.hpp)
+        if (CPLX) Sim[u][l] = fma(im[u], w, Sim[u][l]);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t i = i0 + u;
+    if (i >= d.m) break;
+    const int sc = -(d.expA[i] + d.expB[j]);
+    const double re = ldexp(crt::finish<L>(Sre[u], cc), sc);
+    const double im = CPLX ? ldexp(crt::finish<L>(Sim[u], cc), sc) : 0.0;
+    update_c<CPLX>(d, i, j, re, im);
   }
 }
 
@@ -745,14 +731,32 @@ hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_crt(bool complex, const CrtDesc& d, hipStream_t stream) {
-  const int64_t total = static_cast<int64_t>(d.m) * d.n;
+template <bool CPLX, bool IN8>
+void launch_crt_limbs(const CrtDesc& d, int limbs, dim3 grid, hipStream_t stream) {
+  switch (limbs) {
+    case 1: hipLaunchKernelGGL((crt_kernel<CPLX, IN8, 1>), grid, dim3(256), 0, stream, d); break;
+    case 2: hipLaunchKernelGGL((crt_kernel<CPLX, IN8, 2>), grid, dim3(256), 0, stream, d); break;
+    case 3: hipLaunchKernelGGL((crt_kernel<CPLX, IN8, 3>), grid, dim3(256), 0, stream, d); break;
+    default: hipLaunchKernelGGL((crt_kernel<CPLX, IN8, 4>), grid, dim3(256), 0, stream, d); break;
+  }
+}
+
+hipError_t launch_crt(bool complex, const CrtDesc& d, int limbs, hipStream_t stream) {
+  const int64_t total = (static_cast<int64_t>(d.m) + 3) / 4 * d.n;
   if (total == 0) return hipSuccess;
+  if (limbs < 1 || limbs > kMaxCrtLimbs || d.ldo % 4 != 0) return hipErrorInvalidValue;
   const dim3 grid(static_cast<unsigned>((total + 255) / 256));
-  if (complex)
-    hipLaunchKernelGGL(crt_kernel<true>, grid, dim3(256), 0, stream, d);
-  else
-    hipLaunchKernelGGL(crt_kernel<false>, grid, dim3(256), 0, stream, d);
+  if (complex) {
+    if (d.in8)
+      launch_crt_limbs<true, true>(d, limbs, grid, stream);
+    else
+      launch_crt_limbs<true, false>(d, limbs, grid, stream);
+  } else {
+    if (d.in8)
+      launch_crt_limbs<false, true>(d, limbs, grid, stream);
+    else
+      launch_crt_limbs<false, false>(d, limbs, grid, stream);
+  }
   return hipGetLastError();
 }
 

@@ -30,11 +30,20 @@ struct ModuliSet {
 };
 
 // CRT constants of one moduli set (little-endian 64-bit limbs).
+constexpr int kCrtLimbBits = 40;
+constexpr int kMaxCrtLimbs = 4;  // 160 bits cover 20 moduli of at most 8 bits
+
 struct CrtConstants {
   uint64_t W[kMaxModuli][3];  // W_t = (M / m_t) * ((M / m_t)^-1 mod m_t), W_t < M
   uint64_t M[3];              // product of the moduli
   uint64_t halfM[3];          // floor(M / 2)
+  uint64_t lowM[3];           // M - 1 - floor(M / 2)
   double Md;                  // M rounded to double
+  // 40-bit limbs (crt_arith.hpp): W_t = sum_j Wl[t][j] 2^(40 j), M likewise
+  int limbs;                  // L = ceil(bits(M) / 40)
+  double Wl[kMaxModuli][kMaxCrtLimbs];
+  double Ml[kMaxCrtLimbs];
+  double cq[kMaxCrtLimbs];    // 2^(40 j) / M
 };
 
 // Operand read as `rows` rows of `k` reduction elements.
@@ -62,9 +71,12 @@ struct I8GemmDesc {
   int splits;
   int64_t kSplit;  // reduction bytes per split (multiple of 256)
   int tilesM, tilesN;
-  void* out;       // uint8 residues [G][n][m] (splits == 1) or int32 sums [splits][G'][n][m]
-  int64_t outPlane;  // m * n
-  int64_t outSplitStride;  // int32 sums: elements from one split to the next (G' m n)
+  // uint8 biased residues r + 128 (r in [-128, 127]) [G][n][ldo] (splits == 1)
+  // or int32 sums [splits][G'][n][ldo]
+  void* out;
+  int64_t ldo;       // rows of an output column: a multiple of 4 that covers every tile row
+  int64_t outPlane;  // ldo * n
+  int64_t outSplitStride;  // int32 sums: elements from one split to the next (G' ldo n)
   ModuliSet ms;
   int fillMode;
   int64_t fillRowOff, fillColOff;
@@ -72,11 +84,12 @@ struct I8GemmDesc {
 
 struct CrtDesc {
   const void* W;  // GEMM output (see I8GemmDesc::out)
-  int in8;        // 1: uint8 residues, 0: int32 split sums
+  int in8;        // 1: uint8 biased residues, 0: int32 split sums
   int splits;
   int G;
   int products;
-  int64_t plane;  // m * n
+  int64_t ldo;    // rows of an output column (multiple of 4)
+  int64_t plane;  // ldo * n
   int m, n;
   const int* expA;
   const int* expB;

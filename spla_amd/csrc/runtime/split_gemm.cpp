@@ -199,14 +199,16 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   gd.kSplit = kSplit;
   gd.tilesM = static_cast<int>((m + bm - 1) / bm);
   gd.tilesN = static_cast<int>((n + bn - 1) / bn);
-  gd.outPlane = m * n;
-  gd.outSplitStride = static_cast<int64_t>(G) * m * n;
+  gd.ldo = mpad;  // covers every tile row; a multiple of 4 for word stores
+  gd.outPlane = gd.ldo * n;
+  gd.outSplitStride = static_cast<int64_t>(G) * gd.outPlane;
   gd.ms = ms;
   gd.fillMode = fill.mode;
   gd.fillRowOff = fill.rowOff;
   gd.fillColOff = fill.colOff;
-  const std::size_t outBytes = splits == 1 ? static_cast<std::size_t>(G) * m * n
-                                           : static_cast<std::size_t>(splits) * G * m * n * 4;
+  const std::size_t outBytes = splits == 1
+                                   ? static_cast<std::size_t>(G) * gd.outPlane
+                                   : static_cast<std::size_t>(splits) * G * gd.outPlane * 4;
   keep.emplace_back(ctx.gpu_memory(), outBytes);
   gd.out = keep.back().get();  // tiles skipped by the fill mode are neither written nor read
   {
@@ -222,7 +224,8 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   cd.splits = static_cast<int>(splits);
   cd.G = G;
   cd.products = products;
-  cd.plane = m * n;
+  cd.ldo = gd.ldo;
+  cd.plane = gd.outPlane;
   cd.m = mi;
   cd.n = ni;
   cd.expA = dExpA;
@@ -243,7 +246,7 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   cd.tileCols = bn;
   {
     PhaseScope ps(timer, PhaseSplitCrt, stream);
-    hip_check(launch_crt(cplx, cd, stream));
+    hip_check(launch_crt(cplx, cd, cc.limbs, stream));
   }
   rep.used = 1;
   return true;

@@ -209,7 +209,41 @@ void make_crt_constants(const ModuliSet& ms, CrtConstants& cc) {
   U192 h = M;
   divmod_small(h, 2);
   for (int i = 0; i < 3; ++i) cc.halfM[i] = h.w[i];
+  // lowM = M - 1 - halfM
+  unsigned __int128 borrow = 0;
+  for (int i = 0; i < 3; ++i) {
+    const unsigned __int128 sub = static_cast<unsigned __int128>(h.w[i]) + borrow + (i == 0);
+    const unsigned __int128 mw = M.w[i];
+    cc.lowM[i] = static_cast<uint64_t>(mw - sub);
+    borrow = mw < sub ? 1 : 0;
+  }
   cc.Md = to_double(M);
+  // 40-bit limbs of W_t and M, and 2^(40 j) / M
+  auto limb = [](const U192& v, int j) {
+    const int lo = kCrtLimbBits * j;  // bits lo .. lo + 39 of v
+    uint64_t bits = 0;
+    for (int b = 0; b < kCrtLimbBits; ++b) {
+      const int pos = lo + b;
+      if (pos < 192 && ((v.w[pos / 64] >> (pos % 64)) & 1)) bits |= uint64_t(1) << b;
+    }
+    return static_cast<double>(bits);
+  };
+  int bitsM = 0;
+  for (int pos = 191; pos >= 0; --pos)
+    if ((M.w[pos / 64] >> (pos % 64)) & 1) {
+      bitsM = pos + 1;
+      break;
+    }
+  cc.limbs = std::max(1, (bitsM + kCrtLimbBits - 1) / kCrtLimbBits);
+  for (int t = 0; t < ms.count; ++t) {
+    U192 w;
+    for (int i = 0; i < 3; ++i) w.w[i] = cc.W[t][i];
+    for (int j = 0; j < kMaxCrtLimbs; ++j) cc.Wl[t][j] = limb(w, j);
+  }
+  for (int j = 0; j < kMaxCrtLimbs; ++j) {
+    cc.Ml[j] = limb(M, j);
+    cc.cq[j] = std::ldexp(1.0, kCrtLimbBits * j) / cc.Md;
+  }
 }
 
 }  // namespace spla_amd
