@@ -250,7 +250,8 @@ def side_workload(args) -> int:
     stream = torch.cuda.current_stream()
     ctx = spla.Context(spla.ProcessingUnit.GPU)
     ctx.set_stream(stream)
-    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8}
+    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8,
+             "auto": spla.GemmMode.AUTO}
     steps, warmup = max(1, min(args.steps, 5)), max(1, min(args.warmup, 1))
     out = {"metric": METRIC, "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
            "higher_is_better": True, "data": "synthetic (splitmix64 seed 42)", "forms": {}}

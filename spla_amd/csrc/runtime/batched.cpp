@@ -4,12 +4,15 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <queue>
+#include <type_traits>
 
 #include "kernels/gemm_desc.hpp"
 #include "kernels/launch.hpp"
 #include "runtime/engine.hpp"
 #include "runtime/hip_check.hpp"
+#include "runtime/split_gemm.hpp"
 #include "spla/exceptions.hpp"
 
 namespace spla_amd {
@@ -63,6 +66,77 @@ bool on_device(const void* p) {
     return false;
   }
   return attr.type == hipMemoryTypeDevice;
+}
+
+// Split-precision batch in three rounds over the lanes: the statistics of
+// every block, one synchronization and the host plans of all blocks, then the
+// residue planes, INT8 GEMMs and reconstructions (blocks that fall back run
+// the FP64 kernels). Each lane reuses one workspace sized for its largest
+// block, so the planes of at most one block per lane are held at a time.
+template <typename T>
+void execute_split_batch(spla::ContextInternal& ctx, const std::vector<BlockSpec>& specs,
+                         SplaOperation opA, T alpha, T beta, const hipStream_t* lanes,
+                         int nLanes, std::vector<Block>& keep) {
+  struct Item {
+    const BlockSpec* spec;
+    std::unique_ptr<SplitProduct<T>> p;
+    int lane;
+    bool split;
+  };
+  const bool alphaZero = alpha == T(0);
+  std::vector<Item> items;
+  int next = 0;
+  for (const BlockSpec& b : specs) {
+    if (b.m == 0 || b.n == 0) continue;
+    Item it;
+    it.spec = &b;
+    it.lane = next;
+    next = (next + 1) % nLanes;
+    FillSpec fill;
+    fill.mode = b.fillMode;
+    fill.rowOff = b.rowOffset;
+    fill.colOff = b.colOffset;
+    it.p = std::make_unique<SplitProduct<T>>(
+        opA, SPLA_OP_NONE, b.m, b.n, b.k, alpha, static_cast<const T*>(b.A), b.lda,
+        static_cast<const T*>(b.B), b.ldb, beta, static_cast<T*>(b.C), b.ldc, ctx, fill);
+    // alpha == 0 is beta scaling only, which the FP64 path does without reading A, B
+    it.split = !alphaZero && it.p->enqueue_stats(lanes[it.lane]);
+    items.push_back(std::move(it));
+  }
+  for (int i = 0; i < nLanes; ++i) hip_check(hipStreamSynchronize(lanes[i]));
+  std::vector<std::size_t> laneBytes(nLanes, 0);
+  const bool cplx = !std::is_same<T, double>::value;
+  for (Item& it : items) {
+    if (!it.split) continue;
+    const BlockSpec& b = *it.spec;
+    const double fp64 = ctx.gemm_mode() == GemmModeAuto
+                            ? fp64_estimate_seconds(ctx, cplx, opA, SPLA_OP_NONE, b.m, b.n, b.k)
+                            : 0.0;
+    it.split = it.p->plan(fp64);
+    if (it.split) laneBytes[it.lane] = std::max(laneBytes[it.lane], it.p->workspace_bytes());
+  }
+  std::vector<Block> ws(nLanes);
+  for (int i = 0; i < nLanes; ++i)
+    if (laneBytes[i] > 0) ws[i] = Block(ctx.gpu_memory(), laneBytes[i]);
+  std::vector<Block> reuse(nLanes);
+  for (Item& it : items) {
+    const BlockSpec& b = *it.spec;
+    if (it.split) {
+      it.p->enqueue_rest(lanes[it.lane], ws[it.lane].get(), keep);
+    } else {
+      FillSpec fill;
+      fill.mode = b.fillMode;
+      fill.rowOff = b.rowOffset;
+      fill.colOff = b.colOffset;
+      enqueue_on_device<T>(opA, SPLA_OP_NONE, b.m, b.n, b.k, alpha, static_cast<const T*>(b.A),
+                           b.lda, static_cast<const T*>(b.B), b.ldb, beta, static_cast<T*>(b.C),
+                           b.ldc, ctx, fill, lanes[it.lane], keep, &reuse[it.lane], false);
+    }
+  }
+  if (!items.empty()) ctx.last_split() = items.back().p->report();
+  // workspaces stay alive until the caller has drained the lanes
+  for (Block& w : ws) keep.push_back(std::move(w));
+  for (Block& r : reuse) keep.push_back(std::move(r));
 }
 
 }  // namespace
@@ -325,55 +399,59 @@ void BatchPlan::execute() {
   const hipStream_t s = ctx_->compute_stream();
   ctx_->timer().reset();
   if (!kPersistentLaunch || ctx_->gemm_mode() != GemmModeFP64) {
-    // Block by block on the single-product engine, all enqueued on one stream
-    // and synchronized once (FP64), or one split product at a time (split
-    // precision: every block is its own plan with its own host decision; the
-    // residue planes of a block are released before the next one).
+    // Block by block on the single-product engine. FP64: independent blocks go
+    // round robin to the compute stream and the context's batch streams, so
+    // that small products run side by side, with one synchronization. Split
+    // precision: the same lanes, staged so that all blocks share one host
+    // synchronization for their plans (execute_split_batch).
     const bool split = ctx_->gemm_mode() != GemmModeFP64;
-    // FP64: independent blocks go round robin to the compute stream and the
-    // context's batch streams, so that small products run side by side
     constexpr int kLanes = 1 + spla::ContextInternal::kBatchStreams;
     hipStream_t lanes[kLanes] = {s};
     int nLanes = 1;
-    if (!split) {
-      hipEvent_t start;
-      hip_check(hipEventCreateWithFlags(&start, hipEventDisableTiming));
-      hip_check(hipEventRecord(start, s));
-      for (int i = 0; i < spla::ContextInternal::kBatchStreams; ++i) {
-        lanes[nLanes] = ctx_->batch_stream(i);
-        hip_check(hipStreamWaitEvent(lanes[nLanes++], start, 0));
-      }
-      (void)hipEventDestroy(start);
+    hipEvent_t start;
+    hip_check(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+    hip_check(hipEventRecord(start, s));
+    for (int i = 0; i < spla::ContextInternal::kBatchStreams; ++i) {
+      lanes[nLanes] = ctx_->batch_stream(i);
+      hip_check(hipStreamWaitEvent(lanes[nLanes++], start, 0));
     }
+    (void)hipEventDestroy(start);
     std::vector<Block> keep;
-    Block reuse[kLanes];
-    int next = 0;
-    for (const BlockSpec& b : specs_) {
-      if (b.m == 0 || b.n == 0) continue;
-      const int lane = next;
-      next = (next + 1) % nLanes;
-      const hipStream_t ls = lanes[lane];
-      FillSpec fill;
-      fill.mode = b.fillMode;
-      fill.rowOff = b.rowOffset;
-      fill.colOff = b.colOffset;
+    if (split) {
       if (complex_) {
         using Z = std::complex<double>;
-        enqueue_on_device<Z>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
-                             static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B), b.ldb,
-                             Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_, fill, ls,
-                             keep, &reuse[lane]);
+        execute_split_batch<Z>(*ctx_, specs_, opA_, Z(alpha_[0], alpha_[1]),
+                               Z(beta_[0], beta_[1]), lanes, nLanes, keep);
       } else {
-        enqueue_on_device<double>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
-                                  static_cast<const double*>(b.A), b.lda,
-                                  static_cast<const double*>(b.B), b.ldb, beta_[0],
-                                  static_cast<double*>(b.C), b.ldc, *ctx_, fill, ls, keep,
-                                  &reuse[lane]);
+        execute_split_batch<double>(*ctx_, specs_, opA_, alpha_[0], beta_[0], lanes, nLanes, keep);
       }
-      if (split) {
-        hip_check(hipStreamSynchronize(s));
-        keep.clear();
+    } else {
+      Block reuse[kLanes];
+      int next = 0;
+      for (const BlockSpec& b : specs_) {
+        if (b.m == 0 || b.n == 0) continue;
+        const int lane = next;
+        next = (next + 1) % nLanes;
+        const hipStream_t ls = lanes[lane];
+        FillSpec fill;
+        fill.mode = b.fillMode;
+        fill.rowOff = b.rowOffset;
+        fill.colOff = b.colOffset;
+        if (complex_) {
+          using Z = std::complex<double>;
+          enqueue_on_device<Z>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
+                               static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B),
+                               b.ldb, Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_,
+                               fill, ls, keep, &reuse[lane]);
+        } else {
+          enqueue_on_device<double>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
+                                    static_cast<const double*>(b.A), b.lda,
+                                    static_cast<const double*>(b.B), b.ldb, beta_[0],
+                                    static_cast<double*>(b.C), b.ldc, *ctx_, fill, ls, keep,
+                                    &reuse[lane]);
+        }
       }
+      for (Block& r : reuse) keep.push_back(std::move(r));
     }
     for (int i = 1; i < nLanes; ++i) {  // the compute stream joins the batch streams
       hipEvent_t done;

@@ -110,7 +110,8 @@ template <typename T>
 void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                             T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                             T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
-                            hipStream_t stream, std::vector<Block>& keep, Block* reuse = nullptr) {
+                            hipStream_t stream, std::vector<Block>& keep, Block* reuse = nullptr,
+                            bool trySplit = true) {
   using S = Scalar<T>;
   constexpr bool cplx = S::kComplex;
   GemmDesc d;
@@ -148,7 +149,7 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
   const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
   const GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus());
-  if (ctx.gemm_mode() != GemmModeFP64 &&
+  if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
       enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
                                plan.estSeconds, stream, keep))
     return;
@@ -461,20 +462,29 @@ template <typename T>
 void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                        T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C,
                        int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
-                       hipStream_t stream, std::vector<Block>& keep, Block* reuse) {
+                       hipStream_t stream, std::vector<Block>& keep, Block* reuse,
+                       bool trySplit) {
   enqueue_device_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
-                            stream, keep, reuse);
+                            stream, keep, reuse, trySplit);
+}
+
+double fp64_estimate_seconds(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
+                             SplaOperation opB, int64_t m, int64_t n, int64_t k) {
+  const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
+  const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
+  return ctx.tuner().plan(complex, la, lb, m, n, k, ctx.num_cus()).estSeconds;
 }
 
 template void enqueue_on_device<double>(SplaOperation, SplaOperation, int64_t, int64_t, int64_t,
                                         double, const double*, int64_t, const double*, int64_t,
                                         double, double*, int64_t, spla::ContextInternal&,
-                                        const FillSpec&, hipStream_t, std::vector<Block>&, Block*);
+                                        const FillSpec&, hipStream_t, std::vector<Block>&, Block*,
+                                        bool);
 template void enqueue_on_device<std::complex<double>>(
     SplaOperation, SplaOperation, int64_t, int64_t, int64_t, std::complex<double>,
     const std::complex<double>*, int64_t, const std::complex<double>*, int64_t,
     std::complex<double>, std::complex<double>*, int64_t, spla::ContextInternal&, const FillSpec&,
-    hipStream_t, std::vector<Block>&, Block*);
+    hipStream_t, std::vector<Block>&, Block*, bool);
 
 template void gemm_gpu<float>(SplaOperation, SplaOperation, int64_t, int64_t, int64_t, float,
                               const float*, int64_t, const float*, int64_t, float, float*, int64_t,

@@ -50,179 +50,205 @@ int stats_splits(int layout, int rows, int64_t k) {
 }  // namespace
 
 template <typename T>
-bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
-                           T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
-                           T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
-                           double fp64Seconds, hipStream_t stream, std::vector<Block>& keep) {
+SplitProduct<T>::SplitProduct(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n,
+                              int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                              int64_t ldb, T beta, T* C, int64_t ldc,
+                              spla::ContextInternal& ctx, const FillSpec& fill)
+    : opA_(opA), opB_(opB), m_(m), n_(n), k_(k), alpha_(alpha), beta_(beta), A_(A), B_(B),
+      C_(C), lda_(lda), ldb_(ldb), ldc_(ldc), ctx_(ctx), fill_(fill) {}
+
+template <typename T>
+bool SplitProduct<T>::enqueue_stats(hipStream_t stream) {
+  constexpr bool cplx = Cplx<T>::value;
+  rep_ = SplitReport();
+  if (ctx_.gemm_mode() == GemmModeFP64 || k_ == 0 || m_ == 0 || n_ == 0) {
+    rep_.reason = SplitDisabled;
+    return false;
+  }
+  if (round_up(k_, kKpadAlign) > kMaxKpad) {  // residue rows address with 32-bit offsets
+    rep_.reason = SplitRange;
+    return false;
+  }
+  la_ = (opA_ == SPLA_OP_NONE) ? MNMajor : KMajor;
+  lb_ = (opB_ == SPLA_OP_NONE) ? KMajor : MNMajor;
+  const int mi = static_cast<int>(m_), ni = static_cast<int>(n_);
+  PhaseScope ps(ctx_.timer(), PhaseSplitStats, stream);
+  const int sA = stats_splits(la_, mi, k_), sB = stats_splits(lb_, ni, k_);
+  const std::size_t statDoubles =
+      2 * (static_cast<std::size_t>(sA) * m_ + static_cast<std::size_t>(sB) * n_) +
+      2 * static_cast<std::size_t>(m_ + n_);
+  dStat_ = Block(ctx_.gpu_memory(), statDoubles * sizeof(double));
+  double* partA = dStat_.get<double>();
+  double* partB = partA + 2 * static_cast<std::size_t>(sA) * m_;
+  double* outA = partB + 2 * static_cast<std::size_t>(sB) * n_;  // [max m | sum m | max n | sum n]
+  hip_check(launch_split_stats(cplx, la_, reinterpret_cast<const double*>(A_), lda_, mi, k_,
+                               partA, sA, outA, outA + m_, stream));
+  hip_check(launch_split_stats(cplx, lb_, reinterpret_cast<const double*>(B_), ldb_, ni, k_,
+                               partB, sB, outA + 2 * m_, outA + 2 * m_ + n_, stream));
+  hStat_ = Block(ctx_.pinned_memory(), 2 * static_cast<std::size_t>(m_ + n_) * sizeof(double));
+  hip_check(hipMemcpyAsync(hStat_.get(), outA, hStat_.bytes(), hipMemcpyDeviceToHost, stream));
+  return true;
+}
+
+template <typename T>
+bool SplitProduct<T>::plan(double fp64Seconds) {
   constexpr bool cplx = Cplx<T>::value;
   constexpr int comps = cplx ? 2 : 1;
   constexpr int products = cplx ? 3 : 1;
-  SplitReport& rep = ctx.last_split();
-  rep = SplitReport();
-  const int mode = ctx.gemm_mode();
-  if (mode == GemmModeFP64 || k == 0 || m == 0 || n == 0) {
-    rep.reason = SplitDisabled;
-    return false;
-  }
-  if (round_up(k, kKpadAlign) > kMaxKpad) {  // residue rows address with 32-bit offsets
-    rep.reason = SplitRange;
-    return false;
-  }
-  const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
-  const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
-  const int mi = static_cast<int>(m), ni = static_cast<int>(n);
+  const int mi = static_cast<int>(m_), ni = static_cast<int>(n_);
+  const double* h = hStat_.get<double>();
+  OperandStats stA{h, h + m_, mi, k_, comps};
+  OperandStats stB{h + 2 * m_, h + 2 * m_ + n_, ni, k_, comps};
+  const double tol = std::min(ctx_.split_tolerance(), std::max(fp64_bound(k_, comps), 1e-300));
+  plan_ = plan_split(stA, stB, tol, std::min(ctx_.split_max_moduli(), kMaxModuli));
+  dStat_.reset();
+  hStat_.reset();
+  rep_.reason = plan_.reason;
+  rep_.moduli = plan_.moduli;
+  rep_.bitsA = plan_.PA;
+  rep_.bitsB = plan_.PB;
+  rep_.bound = plan_.bound;
+  if (!plan_.ok) return false;
 
-  // 1. operand statistics
-  PhaseTimer& timer = ctx.timer();
-  const bool timing = timer.enabled();
-  if (timing) timer.begin(PhaseSplitStats, stream);
-  const int sA = stats_splits(la, mi, k), sB = stats_splits(lb, ni, k);
-  const std::size_t statDoubles = 2 * (static_cast<std::size_t>(sA) * m + static_cast<std::size_t>(sB) * n) +
-                                  2 * static_cast<std::size_t>(m + n);
-  keep.emplace_back(ctx.gpu_memory(), statDoubles * sizeof(double));
-  double* dStat = keep.back().get<double>();
-  double* partA = dStat;
-  double* partB = partA + 2 * static_cast<std::size_t>(sA) * m;
-  double* outA = partB + 2 * static_cast<std::size_t>(sB) * n;  // [max m | sum m | max n | sum n]
-  hip_check(launch_split_stats(cplx, la, reinterpret_cast<const double*>(A), lda, mi, k, partA,
-                               sA, outA, outA + m, stream));
-  hip_check(launch_split_stats(cplx, lb, reinterpret_cast<const double*>(B), ldb, ni, k, partB,
-                               sB, outA + 2 * m, outA + 2 * m + n, stream));
-  Block hStat(ctx.pinned_memory(), 2 * static_cast<std::size_t>(m + n) * sizeof(double));
-  hip_check(hipMemcpyAsync(hStat.get(), outA, hStat.bytes(), hipMemcpyDeviceToHost, stream));
-  if (timing) timer.end(stream);
-  hip_check(hipStreamSynchronize(stream));
-  const double* h = hStat.get<double>();
-
-  // 2. plan on the host
-  OperandStats stA{h, h + m, mi, k, comps};
-  OperandStats stB{h + 2 * m, h + 2 * m + n, ni, k, comps};
-  const double tol = std::min(ctx.split_tolerance(), std::max(fp64_bound(k, comps), 1e-300));
-  SplitPlan plan = plan_split(stA, stB, tol, std::min(ctx.split_max_moduli(), kMaxModuli));
-  rep.reason = plan.reason;
-  rep.moduli = plan.moduli;
-  rep.bitsA = plan.PA;
-  rep.bitsB = plan.PB;
-  rep.bound = plan.bound;
-  if (!plan.ok) return false;
-
-  const int G = plan.moduli * products;
-  const int64_t kpad = round_up(k, kKpadAlign);
-  if (mode == GemmModeAuto) {
-    const double ops = 2.0 * G * static_cast<double>(m) * n * kpad;
-    const double bytes = static_cast<double>(m + n) * k * comps * 8.0 * 2 +
-                         static_cast<double>(G) * (m + n) * kpad +
-                         static_cast<double>(G) * m * n * 4.0;
+  G_ = plan_.moduli * products;
+  kpad_ = round_up(k_, kKpadAlign);
+  if (ctx_.gemm_mode() == GemmModeAuto) {
+    const double ops = 2.0 * G_ * static_cast<double>(m_) * n_ * kpad_;
+    const double bytes = static_cast<double>(m_ + n_) * k_ * comps * 8.0 * 2 +
+                         static_cast<double>(G_) * (m_ + n_) * kpad_ +
+                         static_cast<double>(G_) * m_ * n_ * 4.0;
     const double est = ops / kInt8OpsPerSecond + bytes / kHbmBytesPerSecond + 60e-6;
     if (!(est < 0.8 * fp64Seconds)) {
-      rep.reason = SplitNotFaster;
+      rep_.reason = SplitNotFaster;
       return false;
     }
   }
+  mpad_ = round_up(m_, kPlaneRowAlign);
+  npad_ = round_up(n_, kPlaneRowAlign);
+  // INT8 GEMM shape: large tiles unless they leave the chip half empty; the
+  // reduction is split to stay within the INT32 range and to fill the chip
+  const int numCUs = ctx_.num_cus();
+  cfg_ = I8TileL;
+  i8_tile_shape(cfg_, bm_, bn_);
+  int64_t tiles = ((m_ + bm_ - 1) / bm_) * ((n_ + bn_ - 1) / bn_);
+  if (tiles * G_ < 2 * numCUs) {
+    cfg_ = I8TileM;
+    i8_tile_shape(cfg_, bm_, bn_);
+    tiles = ((m_ + bm_ - 1) / bm_) * ((n_ + bn_ - 1) / bn_);
+  }
+  splits_ = (kpad_ + kMaxSplitK - 1) / kMaxSplitK;
+  while (tiles * G_ * splits_ < 2 * numCUs && kpad_ / (splits_ + 1) >= 2048) ++splits_;
+  kSplit_ = round_up((kpad_ + splits_ - 1) / splits_, kKpadAlign);
+  splits_ = (kpad_ + kSplit_ - 1) / kSplit_;
+  return true;
+}
 
-  // 3. exponents and CRT constants to the device
+template <typename T>
+std::size_t SplitProduct<T>::workspace_bytes() const {
+  const std::size_t planes = round_up(static_cast<int64_t>(G_) * mpad_ * kpad_, 256) +
+                             round_up(static_cast<int64_t>(G_) * npad_ * kpad_, 256);
+  const std::size_t outPlane = static_cast<std::size_t>(mpad_) * n_;
+  const std::size_t out = splits_ == 1 ? G_ * outPlane : splits_ * G_ * outPlane * 4;
+  return planes + out;
+}
+
+template <typename T>
+void SplitProduct<T>::enqueue_rest(hipStream_t stream, void* workspace, std::vector<Block>& keep) {
+  constexpr bool cplx = Cplx<T>::value;
+  constexpr int products = cplx ? 3 : 1;
+  const int mi = static_cast<int>(m_), ni = static_cast<int>(n_);
+  PhaseTimer& timer = ctx_.timer();
+
+  // exponents and CRT constants to the device
   ModuliSet ms;
-  make_moduli(plan.moduli, ms);
+  make_moduli(plan_.moduli, ms);
   CrtConstants cc;
   make_crt_constants(ms, cc);
-  const std::size_t expBytes = round_up(static_cast<int64_t>((m + n) * sizeof(int)), 256);
+  const std::size_t expBytes = round_up(static_cast<int64_t>((m_ + n_) * sizeof(int)), 256);
   const std::size_t upBytes = expBytes + sizeof(CrtConstants);
-  keep.emplace_back(ctx.pinned_memory(), upBytes);
+  keep.emplace_back(ctx_.pinned_memory(), upBytes);
   char* hUp = keep.back().get<char>();
-  std::memcpy(hUp, plan.expA.data(), m * sizeof(int));
-  std::memcpy(hUp + m * sizeof(int), plan.expB.data(), n * sizeof(int));
+  std::memcpy(hUp, plan_.expA.data(), m_ * sizeof(int));
+  std::memcpy(hUp + m_ * sizeof(int), plan_.expB.data(), n_ * sizeof(int));
   std::memcpy(hUp + expBytes, &cc, sizeof(cc));
-  keep.emplace_back(ctx.gpu_memory(), upBytes);
+  keep.emplace_back(ctx_.gpu_memory(), upBytes);
   char* dUp = keep.back().get<char>();
   hip_check(hipMemcpyAsync(dUp, hUp, upBytes, hipMemcpyHostToDevice, stream));
   const int* dExpA = reinterpret_cast<const int*>(dUp);
-  const int* dExpB = dExpA + m;
+  const int* dExpB = dExpA + m_;
   const CrtConstants* dCC = reinterpret_cast<const CrtConstants*>(dUp + expBytes);
 
-  // 4. residue planes
-  const int64_t mpad = round_up(m, kPlaneRowAlign), npad = round_up(n, kPlaneRowAlign);
-  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(G) * mpad * kpad);
-  int8_t* RA = keep.back().get<int8_t>();
-  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(G) * npad * kpad);
-  int8_t* RB = keep.back().get<int8_t>();
+  // workspace: residue planes of A and B, INT8 GEMM output
+  if (!workspace) {
+    keep.emplace_back(ctx_.gpu_memory(), workspace_bytes());
+    workspace = keep.back().get();
+  }
+  int8_t* RA = static_cast<int8_t*>(workspace);
+  int8_t* RB = RA + round_up(static_cast<int64_t>(G_) * mpad_ * kpad_, 256);
+  void* out = RB + round_up(static_cast<int64_t>(G_) * npad_ * kpad_, 256);
+
+  // residue planes
   ResidueDesc rd;
   std::memset(&rd, 0, sizeof(rd));
-  rd.X = reinterpret_cast<const double*>(A);
-  rd.ld = lda;
+  rd.X = reinterpret_cast<const double*>(A_);
+  rd.ld = lda_;
   rd.rows = mi;
-  rd.k = k;
-  rd.kpad = kpad;
+  rd.k = k_;
+  rd.kpad = kpad_;
   rd.expo = dExpA;
-  rd.imagSign = (cplx && opA == SPLA_OP_CONJ_TRANSPOSE) ? -1 : 1;
+  rd.imagSign = (cplx && opA_ == SPLA_OP_CONJ_TRANSPOSE) ? -1 : 1;
   rd.R = RA;
-  rd.plane = mpad * kpad;
+  rd.plane = mpad_ * kpad_;
   rd.ms = ms;
-  if (timing) timer.begin(PhaseSplitResidues, stream);
-  hip_check(launch_residues(cplx, la, rd, stream));
-  rd.X = reinterpret_cast<const double*>(B);
-  rd.ld = ldb;
-  rd.rows = ni;
-  rd.expo = dExpB;
-  rd.imagSign = (cplx && opB == SPLA_OP_CONJ_TRANSPOSE) ? -1 : 1;
-  rd.R = RB;
-  rd.plane = npad * kpad;
-  hip_check(launch_residues(cplx, lb, rd, stream));
-  if (timing) timer.end(stream);
-
-  // 5. batched INT8 GEMM
-  const int numCUs = ctx.num_cus();
-  int cfg = I8TileL;
-  int bm, bn;
-  i8_tile_shape(cfg, bm, bn);
-  int64_t tiles = ((m + bm - 1) / bm) * ((n + bn - 1) / bn);
-  if (tiles * G < 2 * numCUs) {
-    cfg = I8TileM;
-    i8_tile_shape(cfg, bm, bn);
-    tiles = ((m + bm - 1) / bm) * ((n + bn - 1) / bn);
+  {
+    PhaseScope ps(timer, PhaseSplitResidues, stream);
+    hip_check(launch_residues(cplx, la_, rd, stream));
+    rd.X = reinterpret_cast<const double*>(B_);
+    rd.ld = ldb_;
+    rd.rows = ni;
+    rd.expo = dExpB;
+    rd.imagSign = (cplx && opB_ == SPLA_OP_CONJ_TRANSPOSE) ? -1 : 1;
+    rd.R = RB;
+    rd.plane = npad_ * kpad_;
+    hip_check(launch_residues(cplx, lb_, rd, stream));
   }
-  int64_t splits = (kpad + kMaxSplitK - 1) / kMaxSplitK;
-  while (tiles * G * splits < 2 * numCUs && kpad / (splits + 1) >= 2048) ++splits;
-  const int64_t kSplit = round_up((kpad + splits - 1) / splits, kKpadAlign);
-  splits = (kpad + kSplit - 1) / kSplit;
+
+  // batched INT8 GEMM
   I8GemmDesc gd;
   std::memset(&gd, 0, sizeof(gd));
   gd.RA = RA;
   gd.RB = RB;
-  gd.planeA = mpad * kpad;
-  gd.planeB = npad * kpad;
-  gd.kpad = kpad;
+  gd.planeA = mpad_ * kpad_;
+  gd.planeB = npad_ * kpad_;
+  gd.kpad = kpad_;
   gd.m = mi;
   gd.n = ni;
-  gd.G = G;
+  gd.G = G_;
   gd.products = products;
-  gd.splits = static_cast<int>(splits);
-  gd.kSplit = kSplit;
-  gd.tilesM = static_cast<int>((m + bm - 1) / bm);
-  gd.tilesN = static_cast<int>((n + bn - 1) / bn);
-  gd.ldo = mpad;  // covers every tile row; a multiple of 4 for word stores
-  gd.outPlane = gd.ldo * n;
-  gd.outSplitStride = static_cast<int64_t>(G) * gd.outPlane;
+  gd.splits = static_cast<int>(splits_);
+  gd.kSplit = kSplit_;
+  gd.tilesM = static_cast<int>((m_ + bm_ - 1) / bm_);
+  gd.tilesN = static_cast<int>((n_ + bn_ - 1) / bn_);
+  gd.ldo = mpad_;  // covers every tile row; a multiple of 4 for word stores
+  gd.outPlane = gd.ldo * n_;
+  gd.outSplitStride = static_cast<int64_t>(G_) * gd.outPlane;
   gd.ms = ms;
-  gd.fillMode = fill.mode;
-  gd.fillRowOff = fill.rowOff;
-  gd.fillColOff = fill.colOff;
-  const std::size_t outBytes = splits == 1
-                                   ? static_cast<std::size_t>(G) * gd.outPlane
-                                   : static_cast<std::size_t>(splits) * G * gd.outPlane * 4;
-  keep.emplace_back(ctx.gpu_memory(), outBytes);
-  gd.out = keep.back().get();  // tiles skipped by the fill mode are neither written nor read
+  gd.fillMode = fill_.mode;
+  gd.fillRowOff = fill_.rowOff;
+  gd.fillColOff = fill_.colOff;
+  gd.out = out;  // tiles skipped by the fill mode are neither written nor read
   {
     PhaseScope ps(timer, PhaseSplitInt8Gemm, stream);
-    hip_check(launch_i8_gemm(gd, cfg, stream));
+    hip_check(launch_i8_gemm(gd, cfg_, stream));
   }
 
-  // 6. reconstruction and alpha / beta update
+  // reconstruction and alpha / beta update
   CrtDesc cd;
   std::memset(&cd, 0, sizeof(cd));
-  cd.W = gd.out;
-  cd.in8 = splits == 1;
-  cd.splits = static_cast<int>(splits);
-  cd.G = G;
+  cd.W = out;
+  cd.in8 = splits_ == 1;
+  cd.splits = static_cast<int>(splits_);
+  cd.G = G_;
   cd.products = products;
   cd.ldo = gd.ldo;
   cd.plane = gd.outPlane;
@@ -232,25 +258,43 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   cd.expB = dExpB;
   cd.cc = dCC;
   cd.ms = ms;
-  cd.C = reinterpret_cast<double*>(C);
-  cd.ldc = ldc;
-  cd.alphaRe = re_of(alpha);
-  cd.alphaIm = im_of(alpha);
-  cd.betaRe = re_of(beta);
-  cd.betaIm = im_of(beta);
+  cd.C = reinterpret_cast<double*>(C_);
+  cd.ldc = ldc_;
+  cd.alphaRe = re_of(alpha_);
+  cd.alphaIm = im_of(alpha_);
+  cd.betaRe = re_of(beta_);
+  cd.betaIm = im_of(beta_);
   cd.betaZero = (cd.betaRe == 0.0 && cd.betaIm == 0.0) ? 1 : 0;
-  cd.fillMode = fill.mode;
-  cd.fillRowOff = fill.rowOff;
-  cd.fillColOff = fill.colOff;
-  cd.tileRows = bm;
-  cd.tileCols = bn;
+  cd.fillMode = fill_.mode;
+  cd.fillRowOff = fill_.rowOff;
+  cd.fillColOff = fill_.colOff;
+  cd.tileRows = bm_;
+  cd.tileCols = bn_;
   {
     PhaseScope ps(timer, PhaseSplitCrt, stream);
     hip_check(launch_crt(cplx, cd, cc.limbs, stream));
   }
-  rep.used = 1;
-  return true;
+  rep_.used = 1;
 }
+
+template <typename T>
+bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
+                           T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
+                           T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
+                           double fp64Seconds, hipStream_t stream, std::vector<Block>& keep) {
+  SplitProduct<T> p(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
+  bool ok = p.enqueue_stats(stream);
+  if (ok) {
+    hip_check(hipStreamSynchronize(stream));
+    ok = p.plan(fp64Seconds);
+  }
+  if (ok) p.enqueue_rest(stream, nullptr, keep);
+  ctx.last_split() = p.report();
+  return ok;
+}
+
+template class SplitProduct<double>;
+template class SplitProduct<std::complex<double>>;
 
 template bool enqueue_split_product<double>(SplaOperation, SplaOperation, int64_t, int64_t,
                                             int64_t, double, const double*, int64_t,

@@ -7,19 +7,69 @@
 #include <cstdint>
 #include <vector>
 
+#include "kernels/ozaki_desc.hpp"
 #include "runtime/allocators.hpp"
 #include "runtime/context_internal.hpp"
+#include "runtime/engine.hpp"
+#include "runtime/split_plan.hpp"
+#include "runtime/split_report.hpp"
 #include "spla/types.h"
 
 namespace spla_amd {
 
-struct FillSpec;
+// One product C <- alpha op(A) op(B) + beta C on device-resident operands in
+// three stages, so that many products can share one host synchronization:
+//   1. enqueue_stats: per-row statistics of both operands on the device,
+//      copied to pinned host memory;
+//   2. plan: after the stream has drained, the host plan (widths, moduli,
+//      exponents) or the reason to fall back to the FP64 kernels;
+//   3. enqueue_rest: residue planes, the batched INT8 GEMM and the
+//      reconstruction with the alpha / beta update.
+template <typename T>
+class SplitProduct {
+public:
+  SplitProduct(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k, T alpha,
+               const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
+               spla::ContextInternal& ctx, const FillSpec& fill);
 
-// Enqueues C <- alpha op(A) op(B) + beta C for device-resident operands on the
-// split-precision path and returns true, or returns false (nothing enqueued)
-// when the plan falls back to the FP64 kernels. The decision needs the operand
-// statistics on the host, so the stream is synchronized once inside. Work
-// buffers are appended to `keep`.
+  // False (nothing enqueued) when the split form does not apply: FP64 mode,
+  // an empty product or a reduction longer than the residue rows allow.
+  bool enqueue_stats(hipStream_t stream);
+  // False when the product falls back to the FP64 kernels (report().reason).
+  bool plan(double fp64Seconds);
+  // Device bytes of the residue planes and INT8 GEMM output (after plan()).
+  std::size_t workspace_bytes() const;
+  // `workspace`: at least workspace_bytes(), 256-byte aligned, or nullptr to
+  // allocate it into `keep`; the other buffers are appended to `keep`, which
+  // must live until the stream has drained.
+  void enqueue_rest(hipStream_t stream, void* workspace, std::vector<Block>& keep);
+
+  const SplitReport& report() const { return rep_; }
+
+private:
+  SplaOperation opA_, opB_;
+  int64_t m_, n_, k_;
+  T alpha_, beta_;
+  const T* A_;
+  const T* B_;
+  T* C_;
+  int64_t lda_, ldb_, ldc_;
+  spla::ContextInternal& ctx_;
+  FillSpec fill_;
+  int la_ = 0, lb_ = 0;
+  Block dStat_, hStat_;
+  SplitPlan plan_;
+  SplitReport rep_;
+  int G_ = 0;
+  int64_t kpad_ = 0, mpad_ = 0, npad_ = 0;
+  int cfg_ = 0, bm_ = 0, bn_ = 0;
+  int64_t splits_ = 1, kSplit_ = 0;
+};
+
+// The single-product form: the three stages with one synchronization of
+// `stream` in between. Returns true when the split form was enqueued, false
+// (nothing but the statistics enqueued) when the product falls back to the
+// FP64 kernels. The report is stored in the context.
 template <typename T>
 bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                            T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
