@@ -45,8 +45,28 @@ struct TileStage {
 
   // base: operand origin (doubles); ld: leading dimension in elements;
   // rowBase/rowLimit: stored-row range; q0/qEnd: embedded reduction range.
+  // A k-block wholly inside both ranges is loaded without per-element checks.
   __device__ __forceinline__ void load(const double* __restrict__ base, int64_t ld, int rowBase,
                                        int rowLimit, int64_t q0, int64_t qEnd, int tid) {
+    if (rowBase + ROWS <= rowLimit && q0 + BK <= qEnd) {
+#pragma unroll
+      for (int u = 0; u < kPerThread; ++u) {
+        int row, qp;
+        coords(tid + u * kThreads, row, qp);
+        const int grow = rowBase + row;
+        const int64_t q = q0 + 2 * qp;
+        if (LAYOUT == KMajor) {
+          const double* p = base + static_cast<int64_t>(grow) * ld * (CPLX ? 2 : 1) + q;
+          regs[u] = dbl2{p[0], p[1]};
+        } else if (CPLX) {
+          const double* p = base + ((q >> 1) * ld + grow) * 2;
+          regs[u] = dbl2{p[0], p[1]};
+        } else {
+          regs[u] = dbl2{base[q * ld + grow], base[(q + 1) * ld + grow]};
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       int row, qp;
