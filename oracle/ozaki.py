@@ -17,6 +17,7 @@ rules of split_plan.cpp so that both can be compared on the same statistics.
 from __future__ import annotations
 
 import math
+import operator
 
 import numpy as np
 
@@ -115,6 +116,38 @@ def limb_to_double(v: int) -> float:
     return -d if v < 0 else d
 
 
+def scaled_row(r: np.ndarray, P: int, cplx: bool):
+    """Integers of one row of op(A) (or column of op(B)) at width P and its exponent:
+    e = P - 1 - floor(log2 max |component|), entries rint(x 2^e) (round half to even)."""
+    comps = np.concatenate([r.real, r.imag]) if cplx else r
+    mx = float(np.max(np.abs(comps))) if comps.size else 0.0
+    e = P - 1 - ilogb(mx) if mx > 0 else 0
+    # ldexp is exact and |x 2^e| <= 2^P <= 2^53, so rint and int64 are exact
+    re = np.rint(np.ldexp(np.asarray(r.real, dtype=np.float64), e)).astype(np.int64).tolist()
+    im = (np.rint(np.ldexp(np.asarray(r.imag, dtype=np.float64), e)).astype(np.int64).tolist()
+          if cplx else None)
+    return re, im, e
+
+
+def emulate_pairs(a_rows, b_cols, PA: int, PB: int):
+    """Entries (i, j) of the split product (alpha = 1, beta = 0) for given rows a_rows[s] of
+    op(A) and columns b_cols[s] of op(B), by the rules of emulate(); one value per pair."""
+    out = []
+    for r, c in zip(a_rows, b_cols):
+        cplx = np.iscomplexobj(r)
+        ar, ai, ea = scaled_row(r, PA, cplx)
+        br, bi, eb = scaled_row(c, PB, cplx)
+        sc = -(ea + eb)
+        if cplx:
+            re = sum(map(operator.mul, ar, br)) - sum(map(operator.mul, ai, bi))
+            im = sum(map(operator.mul, ar, bi)) + sum(map(operator.mul, ai, br))
+            out.append(complex(math.ldexp(limb_to_double(re), sc),
+                               math.ldexp(limb_to_double(im), sc)))
+        else:
+            out.append(math.ldexp(limb_to_double(sum(map(operator.mul, ar, br))), sc))
+    return out
+
+
 def op_rows(X: np.ndarray, ld: int, op: int, rows: int, k: int) -> np.ndarray:
     """rows x k array of op(X) read from column-major storage with leading dimension ld."""
     if op == 0:
@@ -137,12 +170,8 @@ def emulate(opA, opB, m, n, k, A, lda, B, ldb, PA, PB):
     def scaled(rows, P):
         out, expo = [], []
         for r in rows:
-            comps = np.concatenate([r.real, r.imag]) if cplx else r
-            mx = float(np.max(np.abs(comps))) if comps.size else 0.0
-            e = P - 1 - ilogb(mx) if mx > 0 else 0
+            re, im, e = scaled_row(r, P, cplx)
             expo.append(e)
-            re = [int(round(math.ldexp(float(x), e))) for x in r.real]
-            im = [int(round(math.ldexp(float(x), e))) for x in r.imag] if cplx else None
             out.append((re, im))
         return out, expo
 

@@ -97,6 +97,41 @@ def test_split_accuracy(sctx, device, dtype, opA, opB, data):
     assert err <= 1e-13
 
 
+@pytest.mark.parametrize("case", ["z_out8", "z_int32", "d_out8"])
+def test_split_large_tiles_bit_exact(sctx, device, case):
+    """256 x 256 INT8 tiles (the configuration of large products), uint8 residue
+    output (one reduction split) and INT32 split sums (k > 2^17 - 256): sampled
+    entries equal the exact integer restatement bit for bit."""
+    import torch
+
+    cplx = case.startswith("z")
+    m = n = 1024 if cplx else 1792  # tiles x products >= 2 x 256: large tiles
+    k = 140000 if case == "z_int32" else (1500 if cplx else 700)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(4321)
+
+    def operand(rows):
+        x = torch.rand(rows * k * (2 if cplx else 1), generator=gen, device=device,
+                       dtype=torch.float64) * 2 - 1
+        return torch.view_as_complex(x.view(-1, 2)) if cplx else x
+
+    dA, dB = operand(m), operand(n)
+    dC = torch.zeros(m * n, dtype=dA.dtype, device=device)
+    opA = 2 if cplx else 1
+    spla.gemm(opA, 0, m, n, k, 1.0, dA, k, dB, k, 0.0, dC, m, sctx)
+    info = sctx.last_product_info()
+    assert info["split"], info
+    rng = np.random.default_rng(17)
+    rows, cols = rng.integers(0, m, 32), rng.integers(0, n, 32)
+    a = dA.view(m, k)[torch.from_numpy(rows).to(device)].cpu().numpy()
+    b = dB.view(n, k)[torch.from_numpy(cols).to(device)].cpu().numpy()
+    if cplx:
+        a = np.conj(a)
+    want = ozaki.emulate_pairs(list(a), list(b), info["bits_a"], info["bits_b"])
+    got = dC.view(n, m).cpu().numpy()[cols, rows]
+    assert all(g == w for g, w in zip(got, want)), case
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
 def test_split_k_beyond_int32_range(sctx, device, dtype):
     m, n, k = 40, 24, 140000  # > 2^17 - 128: INT32 sums are split

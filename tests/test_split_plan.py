@@ -163,3 +163,22 @@ def test_plan_fallbacks(planner):
     assert not wide["ok"] and wide["reason"] == 3
     few = run_plan(planner, ones, ones * 100, ones, ones * 100, 100, 1, 1e-14, max_moduli=3)
     assert not few["ok"] and few["reason"] == 4
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_emulate_pairs_matches_emulate(cplx):
+    """The sampled-entry form of the restatement equals the full one everywhere."""
+    rng = np.random.default_rng(21 + cplx)
+    m, n, k = 9, 6, 40
+    A = rng.uniform(-1, 1, k * m) * np.exp2(rng.integers(-20, 20, k * m))
+    B = rng.uniform(-1, 1, k * n)
+    if cplx:
+        A = A + 1j * rng.uniform(-1, 1, k * m)
+        B = B + 1j * rng.uniform(-1, 1, k * n)
+    opA = 2 if cplx else 1
+    C = ozaki.emulate(opA, 0, m, n, k, A, k, B, k, 37, 44)
+    opa = ozaki.op_rows(A, k, opA, m, k)
+    opbT = B.reshape(n, k)
+    pairs = [(i, j) for i in range(m) for j in range(n)]
+    got = ozaki.emulate_pairs([opa[i] for i, _ in pairs], [opbT[j] for _, j in pairs], 37, 44)
+    assert all(got[s] == C[i, j] for s, (i, j) in enumerate(pairs))
