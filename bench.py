@@ -48,7 +48,7 @@ INT8_PEAK_TOPS = 5000.0
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes (bench.py
 # --copy-calibration ...), each scaled by what the counter reads for a 1 GiB device
 # copy (reads x 2.0, writes x 1.0). They change only when the kernels change.
-HBM_TRAFFIC_BYTES = {"split": 67.06e9 + 1.51e9, "plain": 244.86e9 + 0.07e9}
+HBM_TRAFFIC_BYTES = {"split": 65.19e9 + 1.51e9, "plain": 139.25e9 + 0.07e9}
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
 
