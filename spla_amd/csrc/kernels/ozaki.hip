@@ -639,7 +639,7 @@ __global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int l = 0; l < L; ++l) Sre[u][l] = Sim[u][l] = 0.0;
-  for (int t = 0; t < d.ms.count; ++t) {
+  auto modulus = [&](int t) {
     double re[4], im[4];
     product_residues<CPLX, IN8>(d, t, off, re, im);
 #pragma unroll
@@ -651,7 +651,8 @@ __global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
         if (CPLX) Sim[u][l] = fma(im[u], w, Sim[u][l]);
       }
     }
-  }
+  };
+  for (int t = 0; t < d.ms.count; ++t) modulus(t);
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int64_t i = i0 + u;

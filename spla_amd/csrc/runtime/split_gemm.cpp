@@ -25,8 +25,10 @@ constexpr int64_t kMaxKpad = int64_t(1) << 24;
 // residue planes hold whole tiles of the INT8 GEMM (rows padded to 256)
 constexpr int64_t kPlaneRowAlign = 256;
 // Sustained rates assumed by the automatic mode.
-constexpr double kInt8OpsPerSecond = 2.0e15;
+constexpr double kInt8OpsPerSecond = 2.0e15;  // 256 x 256 tiles, full chip
 constexpr double kHbmBytesPerSecond = 4.0e12;
+// statistics round trip to the host, plan and the six launches of a product
+constexpr double kSplitOverheadSeconds = 150e-6;
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
@@ -112,17 +114,6 @@ bool SplitProduct<T>::plan(double fp64Seconds) {
 
   G_ = plan_.moduli * products;
   kpad_ = round_up(k_, kKpadAlign);
-  if (ctx_.gemm_mode() == GemmModeAuto) {
-    const double ops = 2.0 * G_ * static_cast<double>(m_) * n_ * kpad_;
-    const double bytes = static_cast<double>(m_ + n_) * k_ * comps * 8.0 * 2 +
-                         static_cast<double>(G_) * (m_ + n_) * kpad_ +
-                         static_cast<double>(G_) * m_ * n_ * 4.0;
-    const double est = ops / kInt8OpsPerSecond + bytes / kHbmBytesPerSecond + 60e-6;
-    if (!(est < 0.8 * fp64Seconds)) {
-      rep_.reason = SplitNotFaster;
-      return false;
-    }
-  }
   mpad_ = round_up(m_, kPlaneRowAlign);
   npad_ = round_up(n_, kPlaneRowAlign);
   // INT8 GEMM shape: large tiles unless they leave the chip half empty; the
@@ -140,6 +131,25 @@ bool SplitProduct<T>::plan(double fp64Seconds) {
   while (tiles * G_ * splits_ < 2 * numCUs && kpad_ / (splits_ + 1) >= 2048) ++splits_;
   kSplit_ = round_up((kpad_ + splits_ - 1) / splits_, kKpadAlign);
   splits_ = (kpad_ + kSplit_ - 1) / kSplit_;
+  if (ctx_.gemm_mode() == GemmModeAuto) {
+    // INT8 work on whole tiles at the rate of the chosen tile, scaled by how
+    // well the workgroups fill the chip (last partial wave), plus the passes
+    // over A, B and C and the host plan between them
+    const double blocks = static_cast<double>(tiles) * G_ * splits_;
+    const double waves = std::ceil(blocks / numCUs);
+    const double fill = blocks / (waves * numCUs);
+    const double ops = 2.0 * G_ * static_cast<double>(((m_ + bm_ - 1) / bm_) * bm_) *
+                       static_cast<double>(((n_ + bn_ - 1) / bn_) * bn_) * kpad_;
+    const double rate = kInt8OpsPerSecond * (cfg_ == I8TileL ? 1.0 : 0.6) * fill;
+    const double bytes = static_cast<double>(m_ + n_) * k_ * comps * 8.0 * 2 +
+                         static_cast<double>(G_) * (m_ + n_) * kpad_ +
+                         static_cast<double>(G_) * m_ * n_ * (splits_ == 1 ? 1.0 : 4.0 * splits_);
+    const double est = ops / rate + bytes / kHbmBytesPerSecond + kSplitOverheadSeconds;
+    if (!(est < 0.8 * fp64Seconds)) {
+      rep_.reason = SplitNotFaster;
+      return false;
+    }
+  }
   return true;
 }
 
