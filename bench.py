@@ -9,7 +9,9 @@ when the timed region starts.
 
 Forms: "plain" (FP64 MFMA kernels) and "split" (FP64-accurate split precision:
 Ozaki scheme II on the INT8 matrix cores, falling back to FP64 when its error
-bound cannot be met).
+bound cannot be met). The split form is planned for a guaranteed normwise error
+bound of 1e-13 (--split-tolerance), the accuracy the metric requires; the
+library's default tolerance is 1e-14 (one more modulus on this workload).
 
 value: FP64-accurate TFLOP/s (8*m*n*k / seconds per step) of the fastest form
 whose measured normwise error ||C - C_dd|| / (||A|| ||B||) (sampled entries
@@ -47,8 +49,9 @@ INT8_PEAK_TOPS = 5000.0
 # HBM bytes (read + write) per launch of the dominant kernel of each form on this
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes (bench.py
 # --copy-calibration ...), each scaled by what the counter reads for a 1 GiB device
-# copy (reads x 2.0, writes x 1.0). They change only when the kernels change.
-HBM_TRAFFIC_BYTES = {"split": 65.19e9 + 1.51e9, "plain": 139.25e9 + 0.07e9}
+# copy (reads x 2.0, writes x 1.0). They change only when the kernels or the split
+# plan (number of moduli) change.
+HBM_TRAFFIC_BYTES = {"split": 51.66e9 + 1.41e9, "plain": 139.97e9 + 0.07e9}
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
 
@@ -403,6 +406,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--forms", default="plain,split", help="comma list of: plain, split")
+    ap.add_argument("--split-tolerance", type=float, default=1e-13,
+                    help="normwise error bound the split form is planned for (the metric's "
+                         "required accuracy; the library default is 1e-14)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -454,6 +460,7 @@ def main() -> int:
     flops = 8.0 * m * n * k
     modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8}
     forms = {}
+    ctx.set_split_tolerance(args.split_tolerance)
     for form in args.forms.split(","):
         ctx.set_gemm_mode(modes[form])
         ctx.set_timing(True)
@@ -540,7 +547,8 @@ def main() -> int:
         "data": "synthetic (splitmix64 seed 42, uniform [-1,1) per component), device-resident",
         "config": {"workload": "spla_pzgemm_ssb A^H B complex double, 1 rank (MPI_COMM_SELF)",
                    "m": m, "n": n, "k": k, "alpha": 1.0, "beta": 0.5,
-                   "distC": "block cyclic 256x256 on 1x1", "form": best_form},
+                   "distC": "block cyclic 256x256 on 1x1", "form": best_form,
+                   "split_tolerance": args.split_tolerance},
         "forms": forms,
         "roofline": roofline,
     }
