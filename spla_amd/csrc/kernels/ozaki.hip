@@ -4,19 +4,21 @@
 //
 //   stats_*      per-row largest |component|, sum of squares, non-finite flag
 //                (HBM bound: reads each operand once);
-//   residue_*    scaled integer rows -> symmetric residues modulo every m_t,
-//                written as K-major int8 planes padded to 256 bytes
-//                (HBM bound: reads the operand once, writes N*products bytes
-//                per element);
+//   residue_*    scaled integer rows -> symmetric residues modulo every m_t
+//                (residue_arith.hpp), written as K-major int8 planes padded to
+//                256 bytes (HBM bound: reads the operand once, writes
+//                N*products bytes per element);
 //   i8_gemm      batched C_g = R_A,g R_B,g^T over all moduli and complex
 //                sub-products in one launch, INT32 accumulation; splits of at
 //                most 2^17 - 256 reduction bytes keep the sums below 2^31
 //                (|residue| <= 128). One split: the epilogue reduces modulo
-//                m_t and stores uint8; several splits: raw int32 sums;
-//   crt          per output element: modular recombination of the complex
-//                sub-products, Chinese remainder reconstruction in 192-bit
-//                integer arithmetic, conversion to FP64, scaling by
-//                2^-(ea_i + eb_j) and the alpha / beta update.
+//                m_t and stores biased bytes r + 128; several splits: raw
+//                int32 sums;
+//   crt          per four output rows: modular recombination of the complex
+//                sub-products, Chinese remainder reconstruction exact in FP64
+//                limb sums and 192-bit integer assembly (crt_arith.hpp),
+//                conversion to FP64, scaling by 2^-(ea_i + eb_j) and the
+//                alpha / beta update.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

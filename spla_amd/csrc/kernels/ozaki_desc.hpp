@@ -6,8 +6,8 @@
 // to integers a, b with |a|, |b| <= 2^P. The integer product a^T b is formed
 // modulo pairwise coprime moduli m_t <= 256 with one INT8 MFMA GEMM per modulus
 // (INT32 accumulation, symmetric residues in [-128, 127]), reconstructed by the
-// Chinese remainder theorem in 192-bit integer arithmetic, converted to FP64
-// and scaled back by 2^-(ea[i] + eb[j]). Complex products use three integer
+// Chinese remainder theorem (exact FP64 limb sums, 192-bit integer assembly),
+// converted to FP64 and scaled back by 2^-(ea[i] + eb[j]). Complex products use three integer
 // products per modulus (T1 = Re a Re b, T2 = Im a Im b,
 // T3 = (Re a + Im a)(Re b + Im b); Re = T1 - T2, Im = T3 - T1 - T2), which is
 // exact in modular arithmetic.
