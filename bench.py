@@ -205,7 +205,8 @@ def cpu_baseline(m, n, k_full, target_seconds=12.0):
     }
 
 
-WORKLOADS = ("ssb2048", "sbs100k", "ssb_real_spread", "batch512", "ssb2048_host")
+WORKLOADS = ("ssb2048", "sbs100k", "ssb_real_spread", "batch512", "ssb2048_host",
+             "ssb2048_c64")
 
 
 def _timed(fn, steps, warmup, stream):
@@ -395,6 +396,39 @@ def side_workload(args) -> int:
         out["config"] = {"workload": "spla_pzgemm_ssb A^H B complex m=n=2048 k=200000, A, B, C in "
                                      "pinned host memory (PCIe-inclusive; not the reported value)"}
         out["dtype"] = "c128"
+
+    elif wl == "ssb2048_c64":
+        # the headline shape in single precision: FP32 matrix cores, FP32 accumulation
+        m, n, k = M, N, K
+        dA = uniform_device(SEED, 1, k * m, True, dev).to(torch.complex64)
+        dB = uniform_device(SEED, 2, k * n, True, dev).to(torch.complex64)
+        dC = torch.zeros(m * n, dtype=torch.complex64, device=dev)
+        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
+                                                                 256, 256)
+        flops = 8.0 * m * n * k
+        rows = np.random.default_rng(3).integers(0, m, 32)
+        cols = np.random.default_rng(4).integers(0, n, 32)
+        a = dA.view(m, k)[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.complex128)
+        b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy().astype(np.complex128)
+        re = _accurate_dot(np.concatenate([a.real, a.imag], -1), np.concatenate([b.real, b.imag], -1))
+        im = _accurate_dot(np.concatenate([a.real, -a.imag], -1), np.concatenate([b.imag, b.real], -1))
+        ref = re + 1j * im  # conj(a) . b
+        nAB = (torch.linalg.vector_norm(dA.to(torch.complex128)).item() *
+               torch.linalg.vector_norm(dB.to(torch.complex128)).item())
+        ctx.set_gemm_mode(spla.GemmMode.FP64)
+        run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, 1.0, dA, k, dB, k,
+                                     0.0, dC, m, 0, 0, dist, ctx)
+        wall, ev = _timed(run, steps, warmup, stream)
+        got = dC.view(n, m).cpu().numpy().astype(np.complex128)[cols, rows]
+        err = float(np.sqrt(np.mean(np.abs((got - ref) / nAB) ** 2) * m * n))
+        out["forms"]["plain"] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
+                                 "device_ms_per_step": round(ev, 3), "normwise_error": err,
+                                 "fraction_of_fp32_matrix_peak": round(flops / wall / 1e9 / 157.3, 4),
+                                 "phase_ms": _phases(ctx, run)}
+        log(f"ssb2048_c64: {out['forms']['plain']}")
+        out["config"] = {"workload": "spla_pcgemm_ssb A^H B complex float m=n=2048 k=200000, "
+                                     "device-resident, FP32 accumulation"}
+        out["dtype"] = "c64"
 
     print(json.dumps(out), flush=True)
     return 0

@@ -45,7 +45,7 @@ __device__ __forceinline__ void update_c(const BatchDesc& d, const BatchBlock& b
 
 template <bool CPLX, int WM, int WN>
 __global__ void __launch_bounds__(kThreads, 2) batched_ssb_kernel(BatchDesc d) {
-  using T = TileMma<CPLX, KMajor, KMajor, WM, WN, 2, 16>;
+  using T = TileMma<double, CPLX, KMajor, KMajor, WM, WN, 2, 16>;
   __shared__ typename T::Smem sm;
   __shared__ int sItem;
   __shared__ int sLast;
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kThreads, 2) batched_ssb_kernel(BatchDesc d) {
     const int col0 = it.tn * T::BN;
     const int64_t kBegin = static_cast<int64_t>(it.s) * b.kSplit;
     const int64_t kEnd = min(b.Kp, kBegin + b.kSplit);
-    const TileOperands op{b.A, b.B, b.lda, b.ldb, b.m, b.n, d.conjA, d.conjB};
+    const TileOperands<double> op{b.A, b.B, b.lda, b.ldb, b.m, b.n, d.conjA, d.conjB};
     dbl4 acc[T::FM][T::FN];
     T::run(op, row0, col0, kBegin, kEnd, sm, acc);
 
@@ -72,8 +72,9 @@ __global__ void __launch_bounds__(kThreads, 2) batched_ssb_kernel(BatchDesc d) {
       continue;
     }
     double* w = d.work + b.workOffset + static_cast<int64_t>(it.s) * b.n * b.Mp;
-    T::for_each(row0, col0, b.m, b.n, acc,
-                [&](int i, int j, double re, double im) { store_work<CPLX>(w, b.Mp, i, j, re, im); });
+    T::for_each(row0, col0, b.m, b.n, acc, [&](int i, int j, double re, double im) {
+      store_work<double, CPLX>(w, b.Mp, i, j, re, im);
+    });
     __threadfence();  // partial sums visible device-wide before the arrival
     __syncthreads();
     unsigned* counter = &d.counters[b.counterBase + it.tm * b.tilesN + it.tn];

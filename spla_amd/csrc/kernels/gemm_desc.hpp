@@ -1,10 +1,12 @@
-// Host/device shared descriptors of the FP64 MFMA GEMM kernels.
+// Host/device shared descriptors of the FP64 and FP32 MFMA GEMM kernels.
 //
 // The kernels compute C <- alpha * opA(A) * opB(B) + beta * C for column-major
-// double or interleaved complex-double operands. A complex product is run as a
-// real product of the "real embedding": every 16-row MFMA fragment holds the
-// real parts (rows 0-7) and the imaginary parts (rows 8-15) of 8 complex rows
-// of C, and the reduction index q = 2*l + c runs over the real (c = 0) and
+// real or interleaved complex operands of one scalar type (double or float;
+// the FP32 kernels accumulate in FP32, as a single-precision BLAS does). A
+// complex product is run as a real product of the "real embedding": every
+// 16-row MFMA fragment holds the real and the imaginary rows of 8 complex rows
+// of C (FP64: rows 0-7 real, 8-15 imaginary; FP32: rows 2c real, 2c + 1
+// imaginary, following each instruction's accumulator layout), and the reduction index q = 2*l + c runs over the real (c = 0) and
 // imaginary (c = 1) parts of A(l, i) / B(l, j). The complex product is then
 // one real GEMM of size (2m) x n x (2k), i.e. exactly 8*m*n*k flops, with no
 // operand expanded in memory. Split-K partial sums are stored as interleaved
@@ -26,10 +28,10 @@ enum OperandLayout : int { KMajor = 0, MNMajor = 1 };
 enum TileFill : int { FillFull = 0, FillUpper = 1, FillLower = 2 };
 
 struct GemmDesc {
-  const double* A;  // first element of A, as doubles
-  const double* B;
-  double* C;
-  double* work;  // split-K partial sums: [splits][N][Mp], raw (unscaled) sums
+  const void* A;  // first element of A (scalars of the kernel's type)
+  const void* B;
+  void* C;
+  void* work;  // split-K partial sums: [splits][N][Mp], raw (unscaled) sums
 
   int64_t lda, ldb, ldc;  // leading dimensions in elements (complex or real)
 

@@ -1,4 +1,6 @@
-// FP64 / complex-FP64 GEMM on the gfx950 FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+// FP64 / complex-FP64 GEMM on the gfx950 FP64 matrix cores (v_mfma_f64_16x16x4_f64),
+// and the same kernels in FP32 / complex-FP32 on the FP32 matrix cores
+// (v_mfma_f32_16x16x4_f32, FP32 accumulation as in a single-precision BLAS).
 //
 // Replaces the vendor ?gemm call of SPLA's local engine
 // (src/gpu_util/multiply_gpu.cpp:40-76 -> rocblas_{d,z}gemm) for the
@@ -7,7 +9,7 @@
 // Structure of one workgroup (256 threads = 4 wave64):
 //   * a BM x BN tile of the embedded output (see gemm_desc.hpp), split over a
 //     WAVES_M x WAVES_N grid of waves, each owning WM x WN = (WM/16)x(WN/16)
-//     MFMA accumulators of 16x16 doubles;
+//     MFMA accumulators of 16x16 scalars;
 //   * the reduction runs over k-blocks of BK embedded elements, double
 //     buffered in LDS with one barrier per k-block; the next block's global
 //     loads are issued before the MFMAs of the current one;
@@ -48,24 +50,26 @@ __device__ __forceinline__ bool tile_active(const GemmDesc& d, int64_t r, int64_
   return true;
 }
 
-template <bool CPLX>
-__device__ __forceinline__ void store_result(const GemmDesc& d, int i, int j, double re,
-                                             double im) {
+// C <- alpha * (re, im) + beta * C in the scalar type S (beta == 0: C not read).
+template <typename S, bool CPLX>
+__device__ __forceinline__ void store_result(const GemmDesc& d, int i, int j, S re, S im) {
+  const S aRe = static_cast<S>(d.alphaRe), aIm = static_cast<S>(d.alphaIm);
+  const S bRe = static_cast<S>(d.betaRe), bIm = static_cast<S>(d.betaIm);
   if (CPLX) {
-    double* c = d.C + (static_cast<int64_t>(j) * d.ldc + i) * 2;
-    double outRe = d.alphaRe * re - d.alphaIm * im;
-    double outIm = d.alphaRe * im + d.alphaIm * re;
+    S* c = static_cast<S*>(d.C) + (static_cast<int64_t>(j) * d.ldc + i) * 2;
+    S outRe = aRe * re - aIm * im;
+    S outIm = aRe * im + aIm * re;
     if (!d.betaZero) {
-      const double cr = c[0], ci = c[1];
-      outRe += d.betaRe * cr - d.betaIm * ci;
-      outIm += d.betaRe * ci + d.betaIm * cr;
+      const S cr = c[0], ci = c[1];
+      outRe += bRe * cr - bIm * ci;
+      outIm += bRe * ci + bIm * cr;
     }
     c[0] = outRe;
     c[1] = outIm;
   } else {
-    double* c = d.C + static_cast<int64_t>(j) * d.ldc + i;
-    double out = d.alphaRe * re;
-    if (!d.betaZero) out += d.betaRe * c[0];
+    S* c = static_cast<S*>(d.C) + static_cast<int64_t>(j) * d.ldc + i;
+    S out = aRe * re;
+    if (!d.betaZero) out += bRe * c[0];
     c[0] = out;
   }
 }
@@ -88,11 +92,12 @@ __device__ __forceinline__ void map_block(const GemmDesc& d, int& s, int& tm, in
   tn = inGroup / gsize;
 }
 
-template <bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK, int MINBLOCKS>
-__global__ void __launch_bounds__(kThreads, MINBLOCKS) gemm_f64_kernel(GemmDesc d) {
-  using T = TileMma<CPLX, LA, LB, WM, WN, WAVES_M, BK>;
-  __shared__ typename T::Smem sm;
-
+// Body of one workgroup (shared by the FP64 and FP32 kernels).
+template <typename S, bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK>
+__device__ __forceinline__ void gemm_tile(const GemmDesc& d,
+                                          typename TileMma<S, CPLX, LA, LB, WM, WN, WAVES_M,
+                                                           BK>::Smem& sm) {
+  using T = TileMma<S, CPLX, LA, LB, WM, WN, WAVES_M, BK>;
   int s, tm, tn;
   map_block(d, s, tm, tn);
   const int row0 = tm * T::BM;  // embedded rows
@@ -105,21 +110,34 @@ __global__ void __launch_bounds__(kThreads, MINBLOCKS) gemm_f64_kernel(GemmDesc 
   }
   const int64_t kBegin = static_cast<int64_t>(s) * d.kSplit;
   const int64_t kEnd = min(d.Kp, kBegin + d.kSplit);
-  const TileOperands op{d.A, d.B, d.lda, d.ldb, d.m, d.n, d.conjA, d.conjB};
-  dbl4 acc[T::FM][T::FN];
+  const TileOperands<S> op{static_cast<const S*>(d.A), static_cast<const S*>(d.B), d.lda, d.ldb,
+                           d.m, d.n, d.conjA, d.conjB};
+  typename T::Acc acc[T::FM][T::FN];
   T::run(op, row0, col0, kBegin, kEnd, sm, acc);
   if (d.splits > 1) {
-    double* w = d.work + static_cast<int64_t>(s) * d.n * d.Mp;
+    S* w = static_cast<S*>(d.work) + static_cast<int64_t>(s) * d.n * d.Mp;
     T::for_each(row0, col0, d.m, d.n, acc,
-                [&](int i, int j, double re, double im) { store_work<CPLX>(w, d.Mp, i, j, re, im); });
+                [&](int i, int j, S re, S im) { store_work<S, CPLX>(w, d.Mp, i, j, re, im); });
   } else {
     T::for_each(row0, col0, d.m, d.n, acc,
-                [&](int i, int j, double re, double im) { store_result<CPLX>(d, i, j, re, im); });
+                [&](int i, int j, S re, S im) { store_result<S, CPLX>(d, i, j, re, im); });
   }
 }
 
+template <bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK, int MINBLOCKS>
+__global__ void __launch_bounds__(kThreads, MINBLOCKS) gemm_f64_kernel(GemmDesc d) {
+  __shared__ typename TileMma<double, CPLX, LA, LB, WM, WN, WAVES_M, BK>::Smem sm;
+  gemm_tile<double, CPLX, LA, LB, WM, WN, WAVES_M, BK>(d, sm);
+}
+
+template <bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK, int MINBLOCKS>
+__global__ void __launch_bounds__(kThreads, MINBLOCKS) gemm_f32_kernel(GemmDesc d) {
+  __shared__ typename TileMma<float, CPLX, LA, LB, WM, WN, WAVES_M, BK>::Smem sm;
+  gemm_tile<float, CPLX, LA, LB, WM, WN, WAVES_M, BK>(d, sm);
+}
+
 // Adds the split-K partial sums in split order and applies alpha / beta.
-template <bool CPLX>
+template <typename S, bool CPLX>
 __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmDesc d, int tileRows,
                                                             int tileCols) {
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
@@ -134,49 +152,67 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmDesc d, int tile
     const int64_t nc = min<int64_t>(tileCols, d.n - c0);
     if (!tile_active(d, r0, c0, nr, nc)) return;
   }
-  double re = 0.0, im = 0.0;
+  S re = S(0), im = S(0);
   const int64_t stride = static_cast<int64_t>(d.n) * d.Mp;
   if (CPLX) {
-    const double* w = d.work + static_cast<int64_t>(j) * d.Mp + 2 * i;
+    const S* w = static_cast<const S*>(d.work) + static_cast<int64_t>(j) * d.Mp + 2 * i;
     for (int s = 0; s < d.splits; ++s) {
       re += w[s * stride];
       im += w[s * stride + 1];
     }
   } else {
-    const double* w = d.work + static_cast<int64_t>(j) * d.Mp + i;
+    const S* w = static_cast<const S*>(d.work) + static_cast<int64_t>(j) * d.Mp + i;
     for (int s = 0; s < d.splits; ++s) re += w[s * stride];
   }
-  store_result<CPLX>(d, i, j, re, im);
+  store_result<S, CPLX>(d, i, j, re, im);
 }
 
 // C <- beta * C (k == 0 or alpha == 0); beta == 0 writes zeros without reading C.
-template <bool CPLX>
+template <typename S, bool CPLX>
 __global__ void __launch_bounds__(256) scale_kernel(GemmDesc d) {
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   const int64_t total = static_cast<int64_t>(d.m) * d.n;
   if (idx >= total) return;
   const int i = static_cast<int>(idx % d.m);
   const int j = static_cast<int>(idx / d.m);
+  const S bRe = static_cast<S>(d.betaRe), bIm = static_cast<S>(d.betaIm);
   if (CPLX) {
-    double* c = d.C + (static_cast<int64_t>(j) * d.ldc + i) * 2;
+    S* c = static_cast<S*>(d.C) + (static_cast<int64_t>(j) * d.ldc + i) * 2;
     if (d.betaZero) {
-      c[0] = 0.0;
-      c[1] = 0.0;
+      c[0] = S(0);
+      c[1] = S(0);
     } else {
-      const double cr = c[0], ci = c[1];
-      c[0] = d.betaRe * cr - d.betaIm * ci;
-      c[1] = d.betaRe * ci + d.betaIm * cr;
+      const S cr = c[0], ci = c[1];
+      c[0] = bRe * cr - bIm * ci;
+      c[1] = bRe * ci + bIm * cr;
     }
   } else {
-    double* c = d.C + static_cast<int64_t>(j) * d.ldc + i;
-    c[0] = d.betaZero ? 0.0 : d.betaRe * c[0];
+    S* c = static_cast<S*>(d.C) + static_cast<int64_t>(j) * d.ldc + i;
+    c[0] = d.betaZero ? S(0) : bRe * c[0];
   }
 }
 
-template <bool CPLX, int LA, int LB>
+template <bool F32, bool CPLX, int LA, int LB>
 hipError_t launch_layout(const GemmDesc& d, int cfg, hipStream_t stream) {
   const dim3 block(kThreads);
   const dim3 grid(static_cast<unsigned>(d.tilesM * d.tilesN * d.splits));
+  if (F32) {
+    switch (cfg) {
+      case TileM:
+        hipLaunchKernelGGL((gemm_f32_kernel<CPLX, LA, LB, 32, 32, 2, 16, 2>), grid, block, 0,
+                           stream, d);
+        break;
+      case TileXL:
+        hipLaunchKernelGGL((gemm_f32_kernel<CPLX, LA, LB, 128, 64, 2, 16, 1>), grid, block, 0,
+                           stream, d);
+        break;
+      default:
+        hipLaunchKernelGGL((gemm_f32_kernel<CPLX, LA, LB, 64, 64, 2, 16, 2>), grid, block, 0,
+                           stream, d);
+        break;
+    }
+    return hipGetLastError();
+  }
   switch (cfg) {
     case TileM:
       hipLaunchKernelGGL((gemm_f64_kernel<CPLX, LA, LB, 32, 32, 2, 16, 2>), grid, block, 0, stream,
@@ -194,43 +230,66 @@ hipError_t launch_layout(const GemmDesc& d, int cfg, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <bool CPLX>
+template <bool F32, bool CPLX>
 hipError_t launch_typed(const GemmDesc& d, int la, int lb, int cfg, hipStream_t stream) {
-  if (la == KMajor && lb == KMajor) return launch_layout<CPLX, KMajor, KMajor>(d, cfg, stream);
-  if (la == KMajor && lb == MNMajor) return launch_layout<CPLX, KMajor, MNMajor>(d, cfg, stream);
-  if (la == MNMajor && lb == KMajor) return launch_layout<CPLX, MNMajor, KMajor>(d, cfg, stream);
-  return launch_layout<CPLX, MNMajor, MNMajor>(d, cfg, stream);
+  if (la == KMajor && lb == KMajor) return launch_layout<F32, CPLX, KMajor, KMajor>(d, cfg, stream);
+  if (la == KMajor && lb == MNMajor)
+    return launch_layout<F32, CPLX, KMajor, MNMajor>(d, cfg, stream);
+  if (la == MNMajor && lb == KMajor)
+    return launch_layout<F32, CPLX, MNMajor, KMajor>(d, cfg, stream);
+  return launch_layout<F32, CPLX, MNMajor, MNMajor>(d, cfg, stream);
 }
 
 }  // namespace
 
 hipError_t launch_gemm_f64(const GemmDesc& d, bool complex, int layoutA, int layoutB, int cfg,
                            hipStream_t stream) {
-  return complex ? launch_typed<true>(d, layoutA, layoutB, cfg, stream)
-                 : launch_typed<false>(d, layoutA, layoutB, cfg, stream);
+  return complex ? launch_typed<false, true>(d, layoutA, layoutB, cfg, stream)
+                 : launch_typed<false, false>(d, layoutA, layoutB, cfg, stream);
 }
 
-hipError_t launch_splitk_reduce(const GemmDesc& d, bool complex, int cfg, hipStream_t stream) {
+hipError_t launch_gemm_f32(const GemmDesc& d, bool complex, int layoutA, int layoutB, int cfg,
+                           hipStream_t stream) {
+  return complex ? launch_typed<true, true>(d, layoutA, layoutB, cfg, stream)
+                 : launch_typed<true, false>(d, layoutA, layoutB, cfg, stream);
+}
+
+template <typename S>
+hipError_t launch_splitk_reduce_s(const GemmDesc& d, bool complex, int cfg, hipStream_t stream) {
   const TileShape ts = tile_shape(cfg);
   const int tileRows = complex ? ts.bm / 2 : ts.bm;
   const int64_t total = static_cast<int64_t>(d.m) * d.n;
   const dim3 grid(static_cast<unsigned>((total + 255) / 256));
   if (complex)
-    hipLaunchKernelGGL(splitk_reduce_kernel<true>, grid, dim3(256), 0, stream, d, tileRows, ts.bn);
+    hipLaunchKernelGGL((splitk_reduce_kernel<S, true>), grid, dim3(256), 0, stream, d, tileRows,
+                       ts.bn);
   else
-    hipLaunchKernelGGL(splitk_reduce_kernel<false>, grid, dim3(256), 0, stream, d, tileRows,
+    hipLaunchKernelGGL((splitk_reduce_kernel<S, false>), grid, dim3(256), 0, stream, d, tileRows,
                        ts.bn);
   return hipGetLastError();
 }
 
-hipError_t launch_scale(const GemmDesc& d, bool complex, hipStream_t stream) {
+hipError_t launch_splitk_reduce(const GemmDesc& d, bool complex, int cfg, hipStream_t stream,
+                                bool f32) {
+  return f32 ? launch_splitk_reduce_s<float>(d, complex, cfg, stream)
+             : launch_splitk_reduce_s<double>(d, complex, cfg, stream);
+}
+
+hipError_t launch_scale(const GemmDesc& d, bool complex, hipStream_t stream, bool f32) {
   const int64_t total = static_cast<int64_t>(d.m) * d.n;
   if (total == 0) return hipSuccess;
   const dim3 grid(static_cast<unsigned>((total + 255) / 256));
-  if (complex)
-    hipLaunchKernelGGL(scale_kernel<true>, grid, dim3(256), 0, stream, d);
-  else
-    hipLaunchKernelGGL(scale_kernel<false>, grid, dim3(256), 0, stream, d);
+  if (f32) {
+    if (complex)
+      hipLaunchKernelGGL((scale_kernel<float, true>), grid, dim3(256), 0, stream, d);
+    else
+      hipLaunchKernelGGL((scale_kernel<float, false>), grid, dim3(256), 0, stream, d);
+  } else {
+    if (complex)
+      hipLaunchKernelGGL((scale_kernel<double, true>), grid, dim3(256), 0, stream, d);
+    else
+      hipLaunchKernelGGL((scale_kernel<double, false>), grid, dim3(256), 0, stream, d);
+  }
   return hipGetLastError();
 }
 

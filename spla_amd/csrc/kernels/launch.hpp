@@ -15,19 +15,17 @@ namespace spla_amd {
 // FP64 / complex-FP64 MFMA GEMM over the tiles described by d (gemm_f64.hip).
 hipError_t launch_gemm_f64(const GemmDesc& d, bool complex, int layoutA, int layoutB, int cfg,
                            hipStream_t stream);
+// The same in FP32 / complex-FP32 (FP32 accumulation; d's pointers are floats).
+hipError_t launch_gemm_f32(const GemmDesc& d, bool complex, int layoutA, int layoutB, int cfg,
+                           hipStream_t stream);
 
-// Fixed-order sum of d.splits partial results in d.work, then alpha / beta.
-hipError_t launch_splitk_reduce(const GemmDesc& d, bool complex, int cfg, hipStream_t stream);
+// Fixed-order sum of d.splits partial results in d.work, then alpha / beta
+// (f32: the scalars of d are floats).
+hipError_t launch_splitk_reduce(const GemmDesc& d, bool complex, int cfg, hipStream_t stream,
+                                bool f32 = false);
 
 // C <- beta * C (beta == 0: C <- 0 without reading C).
-hipError_t launch_scale(const GemmDesc& d, bool complex, hipStream_t stream);
-
-// Column-major 2-D precision conversion; rows / leading dimensions in scalars
-// (a complex column of r elements is 2r scalars) (util.hip).
-hipError_t launch_convert_f32_f64(const float* src, int64_t srcLd, double* dst, int64_t dstLd,
-                                  int64_t rows, int64_t cols, hipStream_t stream);
-hipError_t launch_convert_f64_f32(const double* src, int64_t srcLd, float* dst, int64_t dstLd,
-                                  int64_t rows, int64_t cols, hipStream_t stream);
+hipError_t launch_scale(const GemmDesc& d, bool complex, hipStream_t stream, bool f32 = false);
 
 // Split-precision kernels (ozaki.hip).
 // Per-row largest |component| and sum of squares of an operand read as `rows`

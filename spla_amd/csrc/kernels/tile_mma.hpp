@@ -1,6 +1,14 @@
-// FP64 MFMA tile routine shared by the single-product kernel (gemm_f64.hip)
-// and the persistent batched kernel (batched.hip). Device code only; include
-// from HIP translation units.
+// MFMA tile routine shared by the single-product kernels (gemm_f64.hip: FP64
+// and FP32, real and complex) and the persistent batched kernel (batched.hip).
+// Device code only; include from HIP translation units.
+//
+// The scalar type S is double (v_mfma_f64_16x16x4_f64) or float
+// (v_mfma_f32_16x16x4_f32). Both instructions take lane l's operands as
+// A[l % 16][l / 16] and B[l / 16][l % 16]; their accumulators differ: lane l
+// holds rows (l / 16) + 4 r (FP64) or 4 (l / 16) + r (FP32), r = 0..3, of
+// column l % 16. A complex product puts the real and imaginary rows of a
+// complex row of C into the same lane: FP64 fragment rows r and r + 8, FP32
+// fragment rows 2 c and 2 c + 1.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,26 +22,52 @@ namespace tile {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+
+template <typename S>
+struct Vec;
+template <>
+struct Vec<double> {
+  using V2 = dbl2;
+  using V4 = dbl4;
+};
+template <>
+struct Vec<float> {
+  using V2 = flt2;
+  using V4 = flt4;
+};
 
 constexpr int kThreads = 256;
-constexpr int kPad = 2;  // LDS row padding (doubles): keeps 16-byte alignment, spreads banks
+constexpr int kPad = 2;  // LDS row padding (elements): keeps pair alignment, spreads banks
 
 __device__ __forceinline__ double flip_sign(double x, uint32_t hiMask) {
   uint64_t b = __builtin_bit_cast(uint64_t, x);
   b ^= static_cast<uint64_t>(hiMask) << 32;
   return __builtin_bit_cast(double, b);
 }
+__device__ __forceinline__ float flip_sign(float x, uint32_t mask) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ mask);
+}
+
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 // Stages one operand tile (ROWS stored rows x BK embedded reduction elements)
 // from global memory into registers and from registers into LDS.
 // UNCHECKED: k-blocks wholly inside the operands skip the per-element checks
 // (measured faster on the large tiles, slower on the small ones).
-template <bool CPLX, int LAYOUT, int ROWS, int BK, bool UNCHECKED>
+template <typename S, bool CPLX, int LAYOUT, int ROWS, int BK, bool UNCHECKED>
 struct TileStage {
+  using V2 = typename Vec<S>::V2;
   static constexpr int kPairs = ROWS * BK / 2;
   static constexpr int kPerThread = kPairs / kThreads;
   static_assert(kPairs % kThreads == 0, "tile does not divide over the workgroup");
-  dbl2 regs[kPerThread];
+  V2 regs[kPerThread];
 
   __device__ __forceinline__ static void coords(int pi, int& row, int& qp) {
     if (LAYOUT == KMajor) {
@@ -45,10 +79,10 @@ struct TileStage {
     }
   }
 
-  // base: operand origin (doubles); ld: leading dimension in elements;
+  // base: operand origin (scalars); ld: leading dimension in elements;
   // rowBase/rowLimit: stored-row range; q0/qEnd: embedded reduction range.
   // A k-block wholly inside both ranges is loaded without per-element checks.
-  __device__ __forceinline__ void load(const double* __restrict__ base, int64_t ld, int rowBase,
+  __device__ __forceinline__ void load(const S* __restrict__ base, int64_t ld, int rowBase,
                                        int rowLimit, int64_t q0, int64_t qEnd, int tid) {
     if (UNCHECKED && rowBase + ROWS <= rowLimit && q0 + BK <= qEnd) {
 #pragma unroll
@@ -58,13 +92,13 @@ struct TileStage {
         const int grow = rowBase + row;
         const int64_t q = q0 + 2 * qp;
         if (LAYOUT == KMajor) {
-          const double* p = base + static_cast<int64_t>(grow) * ld * (CPLX ? 2 : 1) + q;
-          regs[u] = dbl2{p[0], p[1]};
+          const S* p = base + static_cast<int64_t>(grow) * ld * (CPLX ? 2 : 1) + q;
+          regs[u] = V2{p[0], p[1]};
         } else if (CPLX) {
-          const double* p = base + ((q >> 1) * ld + grow) * 2;
-          regs[u] = dbl2{p[0], p[1]};
+          const S* p = base + ((q >> 1) * ld + grow) * 2;
+          regs[u] = V2{p[0], p[1]};
         } else {
-          regs[u] = dbl2{base[q * ld + grow], base[(q + 1) * ld + grow]};
+          regs[u] = V2{base[q * ld + grow], base[(q + 1) * ld + grow]};
         }
       }
       return;
@@ -75,15 +109,15 @@ struct TileStage {
       coords(tid + u * kThreads, row, qp);
       const int grow = rowBase + row;
       const int64_t q = q0 + 2 * qp;
-      double d0 = 0.0, d1 = 0.0;
+      S d0 = S(0), d1 = S(0);
       if (grow < rowLimit) {
         if (LAYOUT == KMajor) {
-          const double* p = base + static_cast<int64_t>(grow) * ld * (CPLX ? 2 : 1) + q;
+          const S* p = base + static_cast<int64_t>(grow) * ld * (CPLX ? 2 : 1) + q;
           if (q < qEnd) d0 = p[0];
           if (q + 1 < qEnd) d1 = p[1];
         } else if (CPLX) {
           if (q < qEnd) {
-            const double* p = base + ((q >> 1) * ld + grow) * 2;
+            const S* p = base + ((q >> 1) * ld + grow) * 2;
             d0 = p[0];
             d1 = p[1];
           }
@@ -92,24 +126,25 @@ struct TileStage {
           if (q + 1 < qEnd) d1 = base[(q + 1) * ld + grow];
         }
       }
-      regs[u] = dbl2{d0, d1};
+      regs[u] = V2{d0, d1};
     }
   }
 
-  __device__ __forceinline__ void store(double (*lds)[BK + kPad], int tid) const {
+  __device__ __forceinline__ void store(S (*lds)[BK + kPad], int tid) const {
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       int row, qp;
       coords(tid + u * kThreads, row, qp);
-      *reinterpret_cast<dbl2*>(&lds[row][2 * qp]) = regs[u];
+      *reinterpret_cast<V2*>(&lds[row][2 * qp]) = regs[u];
     }
   }
 };
 
 // Operands of one tile product (element rows m / columns n of C).
+template <typename S>
 struct TileOperands {
-  const double* A;
-  const double* B;
+  const S* A;
+  const S* B;
   int64_t lda, ldb;
   int m, n;
   int conjA, conjB;
@@ -117,10 +152,12 @@ struct TileOperands {
 
 // One BM x BN tile of the embedded product over the embedded reduction range
 // [kBegin, kEnd), accumulated in registers. Shared by the single-product
-// kernel and the persistent batched kernel. On return every thread has passed
+// kernels and the persistent batched kernel. On return every thread has passed
 // the final barrier, so the LDS buffers may be reused at once.
-template <bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK>
+template <typename S, bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK>
 struct TileMma {
+  using V2 = typename Vec<S>::V2;
+  using Acc = typename Vec<S>::V4;
   static constexpr int WAVES_N = 4 / WAVES_M;
   static constexpr int BM = WM * WAVES_M;
   static constexpr int BN = WN * WAVES_N;
@@ -128,15 +165,21 @@ struct TileMma {
   static constexpr int FM = WM / 16;
   static constexpr int FN = WN / 16;
   static_assert(BK % 8 == 0, "k-block must hold whole 8-element sub-blocks");
+  static constexpr bool kF64 = sizeof(S) == 8;
+
+  // Complex fragment row r (= lane % 16 of the A operand): its complex row
+  // inside the fragment and whether it is an imaginary row.
+  __device__ __forceinline__ static int cplx_row(int r) { return kF64 ? (r & 7) : (r >> 1); }
+  __device__ __forceinline__ static bool imag_row(int r) { return kF64 ? r >= 8 : (r & 1) != 0; }
 
   struct Smem {
-    double lA[2][AR][BK + kPad];
-    double lB[2][BN][BK + kPad];
+    S lA[2][AR][BK + kPad];
+    S lB[2][BN][BK + kPad];
   };
 
-  __device__ __forceinline__ static void run(const TileOperands& op, int row0, int col0,
+  __device__ __forceinline__ static void run(const TileOperands<S>& op, int row0, int col0,
                                              int64_t kBegin, int64_t kEnd, Smem& sm,
-                                             dbl4 (&acc)[FM][FN]) {
+                                             Acc (&acc)[FM][FN]) {
     const int arow0 = CPLX ? row0 / 2 : row0;
     const int nk = static_cast<int>((kEnd - kBegin + BK - 1) / BK);
     const int tid = threadIdx.x;
@@ -150,11 +193,11 @@ struct TileMma {
     // Lane-constant complex operand selection (see gemm_desc.hpp):
     //   real row (t = 0): step0 <- Re(a), step1 <- Im(a) * (conjA == conjB ? -1 : 1)
     //   imag row (t = 1): step0 <- Im(a) * (conjA ? -1 : 1), step1 <- Re(a) * (conjB ? -1 : 1)
-    // Fragment row r = lane % 16 is an imaginary row for r >= 8.
+    // Fragment row r = lane % 16 is an imaginary row when imag_row(r).
     bool swapPair = false;
     uint32_t sgn0 = 0, sgn1 = 0;
     if (CPLX) {
-      swapPair = l16 >= 8;
+      swapPair = imag_row(l16);
       if (!swapPair) {
         sgn1 = (op.conjA == op.conjB) ? 0x80000000u : 0u;
       } else {
@@ -166,11 +209,11 @@ struct TileMma {
 #pragma unroll
     for (int f = 0; f < FM; ++f)
 #pragma unroll
-      for (int g = 0; g < FN; ++g) acc[f][g] = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int g = 0; g < FN; ++g) acc[f][g] = Acc{S(0), S(0), S(0), S(0)};
 
     constexpr bool kUnchecked = WM * WN >= 64 * 64;
-    TileStage<CPLX, LA, AR, BK, kUnchecked> stA;
-    TileStage<CPLX, LB, BN, BK, kUnchecked> stB;
+    TileStage<S, CPLX, LA, AR, BK, kUnchecked> stA;
+    TileStage<S, CPLX, LB, BN, BK, kUnchecked> stB;
 
     if (nk > 0) {
       stA.load(op.A, op.lda, arow0, op.m, kBegin, kEnd, tid);
@@ -191,15 +234,15 @@ struct TileMma {
 
 #pragma unroll
       for (int kb8 = 0; kb8 < BK / 8; ++kb8) {
-        double a0[FM], a1[FM];
-        dbl2 b[FN];
+        S a0[FM], a1[FM];
+        V2 b[FN];
 #pragma unroll
         for (int f = 0; f < FM; ++f) {
-          const int sr = CPLX ? (wm * WM) / 2 + 8 * f + (l16 & 7) : wm * WM + 16 * f + l16;
-          const dbl2 v = *reinterpret_cast<const dbl2*>(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
+          const int sr = CPLX ? (wm * WM) / 2 + 8 * f + cplx_row(l16) : wm * WM + 16 * f + l16;
+          const V2 v = *reinterpret_cast<const V2*>(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
           if (CPLX) {
-            const double x = swapPair ? v.y : v.x;
-            const double y = swapPair ? v.x : v.y;
+            const S x = swapPair ? v.y : v.x;
+            const S y = swapPair ? v.x : v.y;
             a0[f] = flip_sign(x, sgn0);
             a1[f] = flip_sign(y, sgn1);
           } else {
@@ -210,18 +253,16 @@ struct TileMma {
 #pragma unroll
         for (int g = 0; g < FN; ++g) {
           const int j = wn * WN + 16 * g + l16;
-          b[g] = *reinterpret_cast<const dbl2*>(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
+          b[g] = *reinterpret_cast<const V2*>(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
         }
 #pragma unroll
         for (int f = 0; f < FM; ++f)
 #pragma unroll
-          for (int g = 0; g < FN; ++g)
-            acc[f][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[f], b[g].x, acc[f][g], 0, 0, 0);
+          for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a0[f], b[g].x, acc[f][g]);
 #pragma unroll
         for (int f = 0; f < FM; ++f)
 #pragma unroll
-          for (int g = 0; g < FN; ++g)
-            acc[f][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[f], b[g].y, acc[f][g], 0, 0, 0);
+          for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a1[f], b[g].y, acc[f][g]);
       }
 
       if (more) {
@@ -233,13 +274,13 @@ struct TileMma {
   }
 
   // Calls f(i, j, re, im) for every element of the tile inside m x n.
-  // Accumulator layout of v_mfma_f64_16x16x4_f64: lane holds rows kk + 4 * r,
-  // r = 0..3, of column lane % 16 (kk = lane / 16). For complex fragments this
-  // is (re, im) of complex rows kk and kk + 4 in (acc[0], acc[2]) and
-  // (acc[1], acc[3]).
+  // Accumulators of lane (kk = lane / 16) of column lane % 16: FP64 rows
+  // kk + 4 r, i.e. (re, im) of complex rows kk and kk + 4 in (acc[0], acc[2])
+  // and (acc[1], acc[3]); FP32 rows 4 kk + r, i.e. (re, im) of complex rows
+  // 2 kk and 2 kk + 1 in (acc[0], acc[1]) and (acc[2], acc[3]).
   template <typename F>
   __device__ __forceinline__ static void for_each(int row0, int col0, int m, int n,
-                                                  const dbl4 (&acc)[FM][FN], F&& f) {
+                                                  const Acc (&acc)[FM][FN], F&& f) {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const int wm = wid % WAVES_M;
@@ -252,26 +293,31 @@ struct TileMma {
       for (int g = 0; g < FN; ++g) {
         const int j = col0 + wn * WN + 16 * g + l16;
         if (j >= n) continue;
-        const dbl4 v = acc[fm][g];
+        const Acc v = acc[fm][g];
         if (CPLX) {
-          const int i0 = row0 / 2 + (wm * WM) / 2 + 8 * fm + kk;
-          if (i0 < m) f(i0, j, v[0], v[2]);
-          if (i0 + 4 < m) f(i0 + 4, j, v[1], v[3]);
+          const int base = row0 / 2 + (wm * WM) / 2 + 8 * fm;
+          if (kF64) {
+            if (base + kk < m) f(base + kk, j, v[0], v[2]);
+            if (base + kk + 4 < m) f(base + kk + 4, j, v[1], v[3]);
+          } else {
+            if (base + 2 * kk < m) f(base + 2 * kk, j, v[0], v[1]);
+            if (base + 2 * kk + 1 < m) f(base + 2 * kk + 1, j, v[2], v[3]);
+          }
         } else {
-          const int p0 = row0 + wm * WM + 16 * fm + kk;
+          const int p0 = row0 + wm * WM + 16 * fm + (kF64 ? kk : 4 * kk);
+          const int step = kF64 ? 4 : 1;
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            if (p0 + 4 * r < m) f(p0 + 4 * r, j, v[r], 0.0);
+            if (p0 + step * r < m) f(p0 + step * r, j, v[r], S(0));
         }
       }
     }
   }
 };
 
-template <bool CPLX>
-__device__ __forceinline__ void store_work(double* w, int64_t Mp, int i, int j, double re,
-                                           double im) {
-  double* p = w + static_cast<int64_t>(j) * Mp + (CPLX ? 2 * i : i);
+template <typename S, bool CPLX>
+__device__ __forceinline__ void store_work(S* w, int64_t Mp, int i, int j, S re, S im) {
+  S* p = w + static_cast<int64_t>(j) * Mp + (CPLX ? 2 * i : i);
   p[0] = re;
   if (CPLX) p[1] = im;
 }

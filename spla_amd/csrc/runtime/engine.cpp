@@ -94,14 +94,34 @@ struct Scalar;
 template <>
 struct Scalar<double> {
   static constexpr bool kComplex = false;
+  static constexpr bool kF32 = false;
+  using Real = double;
   static double re(double a) { return a; }
   static double im(double) { return 0.0; }
 };
 template <>
 struct Scalar<std::complex<double>> {
   static constexpr bool kComplex = true;
+  static constexpr bool kF32 = false;
+  using Real = double;
   static double re(std::complex<double> a) { return a.real(); }
   static double im(std::complex<double> a) { return a.imag(); }
+};
+template <>
+struct Scalar<float> {
+  static constexpr bool kComplex = false;
+  static constexpr bool kF32 = true;
+  using Real = float;
+  static double re(float a) { return a; }
+  static double im(float) { return 0.0; }
+};
+template <>
+struct Scalar<std::complex<float>> {
+  static constexpr bool kComplex = true;
+  static constexpr bool kF32 = true;
+  using Real = float;
+  static double re(std::complex<float> a) { return a.real(); }
+  static double im(std::complex<float> a) { return a.imag(); }
 };
 
 // One product on device-resident operands, enqueued on `stream`. Workspace
@@ -114,11 +134,12 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
                             bool trySplit = true) {
   using S = Scalar<T>;
   constexpr bool cplx = S::kComplex;
+  constexpr bool f32 = S::kF32;
   GemmDesc d;
   std::memset(&d, 0, sizeof(d));
-  d.A = reinterpret_cast<const double*>(A);
-  d.B = reinterpret_cast<const double*>(B);
-  d.C = reinterpret_cast<double*>(C);
+  d.A = A;
+  d.B = B;
+  d.C = C;
   d.lda = lda;
   d.ldb = ldb;
   d.ldc = ldc;
@@ -142,43 +163,49 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
   ctx.last_split() = SplitReport();
   if (k == 0 || (d.alphaRe == 0.0 && d.alphaIm == 0.0)) {
     // BLAS semantics: C <- beta * C, A and B are not read.
-    hip_check(launch_scale(d, cplx, stream));
+    hip_check(launch_scale(d, cplx, stream, f32));
     return;
   }
 
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
   const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
   const GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus());
-  if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
-      enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
-                               plan.estSeconds, stream, keep))
-    return;
+  if constexpr (!f32) {
+    if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
+        enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
+                                 fill, plan.estSeconds, stream, keep))
+      return;
+  }
   d.splits = plan.splits;
   d.kSplit = plan.kSplit;
   d.tilesM = plan.tilesM;
   d.tilesN = plan.tilesN;
   if (plan.splits > 1) {
     const std::size_t bytes =
-        static_cast<std::size_t>(plan.splits) * static_cast<std::size_t>(n) * d.Mp * sizeof(double);
+        static_cast<std::size_t>(plan.splits) * static_cast<std::size_t>(n) * d.Mp *
+        sizeof(typename S::Real);
     if (reuse) {
       // Products enqueued on one stream may share one workspace.
       if (reuse->bytes() < bytes) {
         keep.emplace_back(std::move(*reuse));
         *reuse = Block(ctx.gpu_memory(), bytes);
       }
-      d.work = reuse->get<double>();
+      d.work = reuse->get();
     } else {
       keep.emplace_back(ctx.gpu_memory(), bytes);
-      d.work = keep.back().get<double>();
+      d.work = keep.back().get();
     }
   }
   {
     PhaseScope ps(ctx.timer(), PhaseGemmF64, stream);
-    hip_check(launch_gemm_f64(d, cplx, la, lb, plan.cfg, stream));
+    if (f32)
+      hip_check(launch_gemm_f32(d, cplx, la, lb, plan.cfg, stream));
+    else
+      hip_check(launch_gemm_f64(d, cplx, la, lb, plan.cfg, stream));
   }
   if (plan.splits > 1) {
     PhaseScope ps(ctx.timer(), PhaseSplitKReduce, stream);
-    hip_check(launch_splitk_reduce(d, cplx, plan.cfg, stream));
+    hip_check(launch_splitk_reduce(d, cplx, plan.cfg, stream, f32));
   }
 }
 
@@ -358,65 +385,47 @@ void gemm_gpu_f64(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, in
   hip_check(hipStreamSynchronize(cstream));
 }
 
-// Single precision runs on the FP64 kernels: operands are widened on the
-// device, the product is formed in FP64 and C is rounded back once.
-template <typename TF, typename TD>
-void gemm_gpu_widened(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
-                      TF alpha, const TF* A, int64_t lda, const TF* B, int64_t ldb, TF beta, TF* C,
-                      int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill) {
-  constexpr bool cplx = !std::is_same<TF, float>::value;
+// Single precision on the FP32 matrix cores (FP32 accumulation, as the
+// vendor s/c gemm SPLA calls). Host-resident operands are staged to the device
+// once (packed), C is copied in (beta != 0) and out when it is host memory.
+template <typename TF>
+void gemm_gpu_f32(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
+                  TF alpha, const TF* A, int64_t lda, const TF* B, int64_t ldb, TF beta, TF* C,
+                  int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill) {
   const hipStream_t stream = ctx.compute_stream();
   const int64_t rowsA = (opA == SPLA_OP_NONE) ? m : k, colsA = (opA == SPLA_OP_NONE) ? k : m;
   const int64_t rowsB = (opB == SPLA_OP_NONE) ? k : n, colsB = (opB == SPLA_OP_NONE) ? n : k;
-  const int comps = cplx ? 2 : 1;
   std::vector<Block> keep;
 
-  // Device copy (float) of a possibly host-resident operand, then widening.
-  auto to_device_double = [&](const TF* X, int64_t ldx, int64_t rows, int64_t cols) -> TD* {
-    keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(rows) * cols * sizeof(TD));
-    TD* out = keep.back().get<TD>();
-    if (rows * cols == 0) return out;
-    const float* src = reinterpret_cast<const float*>(X);
-    int64_t srcLd = ldx;
-    if (classify(X) != MemKind::Device) {
-      keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(rows) * cols * sizeof(TF));
-      hip_check(hipMemcpy2DAsync(keep.back().get(), rows * sizeof(TF), X, ldx * sizeof(TF),
-                                 rows * sizeof(TF), cols, hipMemcpyHostToDevice, stream));
-      src = keep.back().get<float>();
-      srcLd = rows;
-    }
-    hip_check(launch_convert_f32_f64(src, srcLd * comps, reinterpret_cast<double*>(out),
-                                     rows * comps, rows * comps, cols, stream));
-    return out;
+  auto on_device = [&](const TF* X, int64_t ldx, int64_t rows, int64_t cols,
+                       int64_t& ldOut) -> const TF* {
+    ldOut = ldx;
+    if (rows * cols == 0 || classify(X) == MemKind::Device) return X;
+    keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(rows) * cols * sizeof(TF));
+    hip_check(hipMemcpy2DAsync(keep.back().get(), rows * sizeof(TF), X, ldx * sizeof(TF),
+                               rows * sizeof(TF), cols, hipMemcpyHostToDevice, stream));
+    ldOut = rows;
+    return keep.back().get<TF>();
   };
 
-  const TD* dA = k > 0 ? to_device_double(A, lda, rowsA, colsA) : nullptr;
-  const TD* dB = k > 0 ? to_device_double(B, ldb, rowsB, colsB) : nullptr;
-  const bool betaZero = (beta == TF(0));
-  TD* dC;
+  int64_t ldA = lda, ldB = ldb;
+  const TF* dA = k > 0 ? on_device(A, lda, rowsA, colsA, ldA) : A;
+  const TF* dB = k > 0 ? on_device(B, ldb, rowsB, colsB, ldB) : B;
   const bool hostC = classify(C) != MemKind::Device;
-  float* cF = reinterpret_cast<float*>(C);
-  int64_t cFLd = ldc;
+  TF* dC = C;
+  int64_t ldC = ldc;
   if (hostC) {
     keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(m) * n * sizeof(TF));
-    cF = keep.back().get<float>();
-    cFLd = m;
-    if (!betaZero)
-      hip_check(hipMemcpy2DAsync(cF, m * sizeof(TF), C, ldc * sizeof(TF), m * sizeof(TF), n,
+    dC = keep.back().get<TF>();
+    ldC = m;
+    if (!(beta == TF(0)))
+      hip_check(hipMemcpy2DAsync(dC, m * sizeof(TF), C, ldc * sizeof(TF), m * sizeof(TF), n,
                                  hipMemcpyHostToDevice, stream));
   }
-  keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(m) * n * sizeof(TD));
-  dC = keep.back().get<TD>();
-  if (!betaZero)
-    hip_check(launch_convert_f32_f64(cF, cFLd * comps, reinterpret_cast<double*>(dC), m * comps,
-                                     m * comps, n, stream));
-  const TD alphaD(alpha), betaD(beta);
-  enqueue_device_product<TD>(opA, opB, m, n, k, alphaD, dA, rowsA, dB, rowsB, betaD, dC, m, ctx,
-                             fill, stream, keep);
-  hip_check(launch_convert_f64_f32(reinterpret_cast<const double*>(dC), m * comps, cF,
-                                   cFLd * comps, m * comps, n, stream));
+  enqueue_device_product<TF>(opA, opB, m, n, k, alpha, dA, ldA, dB, ldB, beta, dC, ldC, ctx, fill,
+                             stream, keep);
   if (hostC)
-    hip_check(hipMemcpy2DAsync(C, ldc * sizeof(TF), cF, m * sizeof(TF), m * sizeof(TF), n,
+    hip_check(hipMemcpy2DAsync(C, ldc * sizeof(TF), dC, m * sizeof(TF), m * sizeof(TF), n,
                                hipMemcpyDeviceToHost, stream));
   hip_check(hipStreamSynchronize(stream));
 }
@@ -447,12 +456,8 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
     }
   } resolveOnExit{ctx};
 
-  if constexpr (std::is_same<T, float>::value) {
-    gemm_gpu_widened<float, double>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
-                                    fill);
-  } else if constexpr (std::is_same<T, std::complex<float>>::value) {
-    gemm_gpu_widened<std::complex<float>, std::complex<double>>(opA, opB, m, n, k, alpha, A, lda,
-                                                                B, ldb, beta, C, ldc, ctx, fill);
+  if constexpr (Scalar<T>::kF32) {
+    gemm_gpu_f32<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
   } else {
     gemm_gpu_f64<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
   }
