@@ -203,6 +203,33 @@ SPLA_EXPORT SplaError spla_amd_ctx_set_timing(SplaContext ctx, int enable);
  */
 SPLA_EXPORT SplaError spla_amd_ctx_last_timing(SplaContext ctx, double* ms);
 
+/*
+ * Process-wide wall-clock timer. SPLA exports these four functions (without a
+ * header) when built with SPLA_TIMING (src/timing/timing.cpp:49-86); the
+ * Fortran and test code of SPLA call them with (length, pointer) strings.
+ * Scopes nest per thread: a scope started while another is open on the same
+ * thread becomes its child, and repeated scopes at one place accumulate their
+ * durations (seconds). A stop that does not match the innermost open scope is
+ * ignored and reported as a warning by spla_timer_print.
+ */
+SPLA_EXPORT SplaError spla_timer_start(int n, const char* name);
+SPLA_EXPORT SplaError spla_timer_stop(int n, const char* name);
+/* Writes {"name": {"timings": [...], "sub-timings": {...}}, ...} to the file
+ * named by (n, name); SPLA_INVALID_PARAMETER_ERROR if it cannot be opened. */
+SPLA_EXPORT SplaError spla_timer_export_json(int n, const char* name);
+/* Prints a table (count, total, % of the top-level scope, % of the parent,
+ * median, min, max) to standard output. */
+SPLA_EXPORT SplaError spla_timer_print(void);
+
+/*
+ * Records the library's own scopes ("pgemm_ssb_gpu", "pgemm_sbs_gpu" as in
+ * SPLA, "gemm_gpu" around every local product and "split_plan" around the host
+ * plan of a split-precision product) into the same timer. Off by default.
+ */
+SPLA_EXPORT SplaError spla_amd_timer_set_library_scopes(int enable);
+/* Drops every recorded timing and warning; scopes still open are discarded. */
+SPLA_EXPORT SplaError spla_amd_timer_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,4 +40,5 @@ from spla_amd.api import (  # noqa: F401
     pgemm_ssbtr,
     region_pieces,
     ssb_ring_blocks,
+    timer,
 )

@@ -106,6 +106,12 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("spla_amd_ctx_set_split_tolerance", c_void_p, c_double)
     fn("spla_amd_ctx_set_timing", c_void_p, c_int)
     fn("spla_amd_ctx_last_timing", c_void_p, P(c_double))
+    fn("spla_timer_start", c_int, ctypes.c_char_p)
+    fn("spla_timer_stop", c_int, ctypes.c_char_p)
+    fn("spla_timer_export_json", c_int, ctypes.c_char_p)
+    fn("spla_timer_print")
+    fn("spla_amd_timer_set_library_scopes", c_int)
+    fn("spla_amd_timer_clear")
     fn("spla_amd_batch_create_ssb", P(c_void_p), c_void_p, c_int, c_int, c_int, c_void_p,
        c_void_p, c_void_p, c_int)
     fn("spla_amd_batch_execute", c_void_p)

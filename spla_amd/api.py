@@ -401,6 +401,37 @@ _PREFIX = {np.dtype(np.float32): "s", np.dtype(np.float64): "d",
            np.dtype(np.complex64): "c", np.dtype(np.complex128): "z"}
 
 
+class timer:  # noqa: N801 - mirrors SPLA's spla_timer_* family
+    """Process-wide wall-clock timer (SPLA src/timing/timing.cpp:49-86)."""
+
+    @staticmethod
+    def start(name: str) -> None:
+        b = name.encode()
+        check(_lib.lib().spla_timer_start(len(b), b))
+
+    @staticmethod
+    def stop(name: str) -> None:
+        b = name.encode()
+        check(_lib.lib().spla_timer_stop(len(b), b))
+
+    @staticmethod
+    def export_json(path: str) -> None:
+        b = str(path).encode()
+        check(_lib.lib().spla_timer_export_json(len(b), b))
+
+    @staticmethod
+    def print() -> None:
+        check(_lib.lib().spla_timer_print())
+
+    @staticmethod
+    def set_library_scopes(enable: bool) -> None:
+        check(_lib.lib().spla_amd_timer_set_library_scopes(1 if enable else 0))
+
+    @staticmethod
+    def clear() -> None:
+        check(_lib.lib().spla_amd_timer_clear())
+
+
 def _dtype_of(x):
     if isinstance(x, np.ndarray):
         return x.dtype

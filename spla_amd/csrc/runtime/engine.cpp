@@ -12,6 +12,7 @@
 #include "kernels/launch.hpp"
 #include "runtime/hip_check.hpp"
 #include "runtime/split_gemm.hpp"
+#include "runtime/timer.hpp"
 #include "spla/exceptions.hpp"
 
 namespace spla_amd {
@@ -437,6 +438,7 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
               const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
               spla::ContextInternal& ctx, FillSpec fill) {
   if (m == 0 || n == 0) return;
+  timer::LibraryScope scope("gemm_gpu");
   check_gemm_param(opA, opB, m, n, k, A, lda, B, ldb, C, ldc);
   if (opA != SPLA_OP_NONE && opA != SPLA_OP_TRANSPOSE && opA != SPLA_OP_CONJ_TRANSPOSE)
     throw spla::InvalidParameterError();

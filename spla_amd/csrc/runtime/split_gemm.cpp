@@ -10,6 +10,7 @@
 #include "runtime/engine.hpp"
 #include "runtime/hip_check.hpp"
 #include "runtime/split_plan.hpp"
+#include "runtime/timer.hpp"
 
 namespace spla_amd {
 namespace {
@@ -295,6 +296,7 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
   SplitProduct<T> p(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
   bool ok = p.enqueue_stats(stream);
   if (ok) {
+    timer::LibraryScope scope("split_plan");  // statistics wait and the host plan
     hip_check(hipStreamSynchronize(stream));
     ok = p.plan(fp64Seconds);
   }
