@@ -427,68 +427,84 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
     for (int u = 0; u < CB; ++u)
       *reinterpret_cast<int4v*>(&lB[buf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = rb[u];
   };
-  // One k-block step over LDS buffer buf. Its 64 MFMAs (4 steps of 32 bytes x
-  // 16) are issued in groups of 4; between the groups go, spread evenly: the
-  // 16 global loads of k-block kb into register set (la, lb), the LDS operand
-  // reads of the next 32-byte step, and (during the last two steps) the 16 LDS
-  // writes of register set (sa, sb) into buffer sbuf. Spreading the memory
-  // instructions keeps their queues short, so they never hold up MFMA issue.
+  // One k-block step over LDS buffer buf: 4 sub-steps of 32 bytes, FM x FN
+  // MFMAs each. A single wave per SIMD issues at most one instruction every few
+  // cycles, so memory instructions are spread one per MFMA (never in bursts
+  // that would leave the matrix pipe idle while they issue):
+  //   * LDS reads of the next sub-step's fragments follow the first FM + FN
+  //     MFMAs, B first and then A in the order the MFMAs consume them;
+  //   * global loads of k-block kb into register set (la, lb), each with its
+  //     address computation, follow every other MFMA of the first half;
+  //   * the LDS writes of register set (sa, sb) into buffer sbuf follow the last
+  //     MFMAs of sub-step 2 and the first of sub-step 3, leaving them half a
+  //     sub-step to complete before the barrier.
+  // The first sub-step's fragments are read right after the barrier.
   auto step = [&](int buf, int4v (&la)[CA], int4v (&lb)[CB], int kb, const int4v (&sa)[CA],
                   const int4v (&sb)[CB], int sbuf) {
-    // groups of FN MFMAs: FM per 32-byte step, 4 FM per k-block; one global load per
-    // group, two LDS writes per group of the last two steps
-    static_assert(FM == FN && CA == CB && CA + CB == 4 * FM && BK == 128, "tile schedule");
+    constexpr int kSub = BK / 32;          // sub-steps of 32 bytes
+    constexpr int kMfma = FM * FN;         // MFMAs per sub-step
+    constexpr int kReads = FM + FN;        // LDS fragment reads per sub-step
+    constexpr int kLoads = (CA + CB) / kSub;  // global loads per sub-step
+    constexpr int kHalfWrites = (CA + CB) / 2;
+    static_assert(FM == FN && CA == CB && kSub == 4 && kReads <= kMfma && 2 * kLoads <= kMfma &&
+                      kHalfWrites <= kMfma,
+                  "tile schedule");
     const int64_t k0 = kBegin + static_cast<int64_t>(min(kb, nk - 1)) * BK;
     const int8_t* Ak = At + k0;
     const int8_t* Bk = Bt + k0;
     int4v a[2][FM], b[2][FN];
+    auto read_frags = [&](int set, int ks) {
 #pragma unroll
-    for (int f = 0; f < FM; ++f)
-      a[0][f] = *reinterpret_cast<const int4v*>(&lA[buf][wm * WM + 32 * f + l32][16 * khalf]);
+      for (int q = 0; q < FN; ++q)
+        b[set][q] = *reinterpret_cast<const int4v*>(
+            &lB[buf][wn * WN + 32 * q + l32][ks * 32 + 16 * khalf]);
 #pragma unroll
-    for (int q = 0; q < FN; ++q)
-      b[0][q] = *reinterpret_cast<const int4v*>(&lB[buf][wn * WN + 32 * q + l32][16 * khalf]);
-    __builtin_amdgcn_sched_barrier(0);
+      for (int f = 0; f < FM; ++f)
+        a[set][f] = *reinterpret_cast<const int4v*>(
+            &lA[buf][wm * WM + 32 * f + l32][ks * 32 + 16 * khalf]);
+    };
+    read_frags(0, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, kReads, 0);  // LDS reads
 #pragma unroll
-    for (int ks = 0; ks < BK / 32; ++ks) {
+    for (int ks = 0; ks < kSub; ++ks) {
       const int cur = ks & 1;
 #pragma unroll
-      for (int h = 0; h < FM; ++h) {
+      for (int h = 0; h < FM; ++h)
 #pragma unroll
         for (int q = 0; q < FN; ++q)
           acc[h][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cur][h], b[cur][q], acc[h][q], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
-        const int gi = FM * ks + h;  // global load of this group
+      if (ks + 1 < kSub) read_frags(cur ^ 1, ks + 1);
+#pragma unroll
+      for (int j = 0; j < kLoads; ++j) {
+        const int gi = kLoads * ks + j;
         if (gi < CA)
           la[gi] = *reinterpret_cast<const int4v*>(Ak + voff[gi]);
         else
           lb[gi - CA] = *reinterpret_cast<const int4v*>(Bk + voff[gi - CA]);
-        if (ks + 1 < BK / 32) {
-          // next step's operands, B fragments first: the first group of the next
-          // step needs all of B but only the first A fragment
+      }
+      if (ks >= kSub - 2) {
 #pragma unroll
-          for (int e = 2 * h; e < 2 * h + 2; ++e) {
-            if (e < FN)
-              b[cur ^ 1][e] = *reinterpret_cast<const int4v*>(
-                  &lB[buf][wn * WN + 32 * e + l32][(ks + 1) * 32 + 16 * khalf]);
-            else
-              a[cur ^ 1][e - FN] = *reinterpret_cast<const int4v*>(
-                  &lA[buf][wm * WM + 32 * (e - FN) + l32][(ks + 1) * 32 + 16 * khalf]);
-          }
+        for (int j = 0; j < kHalfWrites; ++j) {
+          const int u = kHalfWrites * (ks - (kSub - 2)) + j;
+          if (u < CA)
+            *reinterpret_cast<int4v*>(&lA[sbuf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = sa[u];
+          else
+            *reinterpret_cast<int4v*>(&lB[sbuf][(tid >> 3) + 32 * (u - CA)][16 * (tid & 7)]) =
+                sb[u - CA];
         }
-        if (ks >= BK / 32 - 2) {
-          const int wi = 2 * (FM * (ks - (BK / 32 - 2)) + h);  // 0, 2, .., CA + CB - 2
+      }
+      // issue order of this sub-step (masks: 0x008 MFMA, 0x100 LDS read,
+      // 0x002 VALU, 0x020 global read, 0x200 LDS write)
 #pragma unroll
-          for (int e = 0; e < 2; ++e) {
-            const int u = wi + e;
-            if (u < CA)
-              *reinterpret_cast<int4v*>(&lA[sbuf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = sa[u];
-            else
-              *reinterpret_cast<int4v*>(&lB[sbuf][(tid >> 3) + 32 * (u - CA)][16 * (tid & 7)]) =
-                  sb[u - CA];
-          }
+      for (int i = 0; i < kMfma; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (ks + 1 < kSub && i < kReads) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i % 2 == 0 && i / 2 < kLoads) {
+          __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if ((ks == kSub - 2 && i >= kMfma - kHalfWrites) || (ks == kSub - 1 && i < kHalfWrites))
+          __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
       }
     }
   };
