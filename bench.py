@@ -439,7 +439,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="plain,split", help="comma list of: plain, split")
+    ap.add_argument("--forms", default="plain,split", help="comma list of: plain, split, auto")
     ap.add_argument("--split-tolerance", type=float, default=1e-13,
                     help="normwise error bound the split form is planned for (the metric's "
                          "required accuracy; the library default is 1e-14)")
@@ -492,7 +492,8 @@ def main() -> int:
         del src, dst
 
     flops = 8.0 * m * n * k
-    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8}
+    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8,
+             "auto": spla.GemmMode.AUTO}
     forms = {}
     ctx.set_split_tolerance(args.split_tolerance)
     for form in args.forms.split(","):
@@ -529,7 +530,7 @@ def main() -> int:
                  "device_ms_per_step": round(kernel_ms, 3),
                  "phase_ms_per_step": {p: round(v / args.steps, 4)
                                        for p, v in phase_sum.items() if v > 0}}
-        if form == "split":
+        if form in ("split", "auto"):
             entry["split_used"] = info["split"]
             entry["moduli"] = info["moduli"]
             entry["bits"] = [info["bits_a"], info["bits_b"]]
@@ -547,7 +548,7 @@ def main() -> int:
     ok = {f: e for f, e in forms.items() if e.get("accurate", True)}
     best_form = max(ok, key=lambda f: ok[f]["tflops"]) if ok else "plain"
     best = forms[best_form]
-    if best_form == "split" and best.get("split_used"):
+    if best_form in ("split", "auto") and best.get("split_used"):
         # dominant kernel: the batched INT8 GEMM; algorithmic ops = 2 m n k per product
         # (3 complex sub-products per modulus; the zero padding of k is not counted)
         ops = 2.0 * m * n * k * 3 * best["moduli"]

@@ -99,8 +99,14 @@ void execute_split_batch(spla::ContextInternal& ctx, const std::vector<BlockSpec
     it.p = std::make_unique<SplitProduct<T>>(
         opA, SPLA_OP_NONE, b.m, b.n, b.k, alpha, static_cast<const T*>(b.A), b.lda,
         static_cast<const T*>(b.B), b.ldb, beta, static_cast<T*>(b.C), b.ldc, ctx, fill);
-    // alpha == 0 is beta scaling only, which the FP64 path does without reading A, B
-    it.split = !alphaZero && it.p->enqueue_stats(lanes[it.lane]);
+    // alpha == 0 is beta scaling only, which the FP64 path does without reading A, B;
+    // in automatic mode blocks the split form cannot win skip the statistics
+    const bool auto_ = ctx.gemm_mode() == GemmModeAuto;
+    it.split = !alphaZero &&
+               (!auto_ || it.p->may_beat_fp64(fp64_estimate_seconds(
+                              ctx, !std::is_same<T, double>::value, opA, SPLA_OP_NONE, b.m, b.n,
+                              b.k))) &&
+               it.p->enqueue_stats(lanes[it.lane]);
     items.push_back(std::move(it));
   }
   for (int i = 0; i < nLanes; ++i) hip_check(hipStreamSynchronize(lanes[i]));

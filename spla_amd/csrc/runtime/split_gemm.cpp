@@ -94,6 +94,26 @@ bool SplitProduct<T>::enqueue_stats(hipStream_t stream) {
 }
 
 template <typename T>
+bool SplitProduct<T>::may_beat_fp64(double fp64Seconds) {
+  constexpr bool cplx = Cplx<T>::value;
+  constexpr int comps = cplx ? 2 : 1;
+  constexpr int products = cplx ? 3 : 1;
+  if (m_ == 0 || n_ == 0 || k_ == 0) return true;
+  const double tol = std::min(ctx_.split_tolerance(), std::max(fp64_bound(k_, comps), 1e-300));
+  const double G = static_cast<double>(min_moduli(tol)) * products;
+  const double kpad = static_cast<double>(round_up(k_, kKpadAlign));
+  const double md = static_cast<double>(m_), nd = static_cast<double>(n_);
+  // the cost plan() estimates, with no tile padding, a full chip and one split
+  const double ops = 2.0 * G * md * nd * kpad;
+  const double bytes = (md + nd) * static_cast<double>(k_) * comps * 8.0 * 2 + G * (md + nd) * kpad +
+                       G * md * nd;
+  const double est = ops / kInt8OpsPerSecond + bytes / kHbmBytesPerSecond + kSplitOverheadSeconds;
+  if (est < 0.8 * fp64Seconds) return true;
+  rep_.reason = SplitNotFaster;
+  return false;
+}
+
+template <typename T>
 bool SplitProduct<T>::plan(double fp64Seconds) {
   constexpr bool cplx = Cplx<T>::value;
   constexpr int comps = cplx ? 2 : 1;
@@ -294,7 +314,8 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
                            T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
                            double fp64Seconds, hipStream_t stream, std::vector<Block>& keep) {
   SplitProduct<T> p(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
-  bool ok = p.enqueue_stats(stream);
+  bool ok = (ctx.gemm_mode() != GemmModeAuto || p.may_beat_fp64(fp64Seconds)) &&
+            p.enqueue_stats(stream);
   if (ok) {
     timer::LibraryScope scope("split_plan");  // statistics wait and the host plan
     hip_check(hipStreamSynchronize(stream));

@@ -35,6 +35,11 @@ public:
   // False (nothing enqueued) when the split form does not apply: FP64 mode,
   // an empty product or a reduction longer than the residue rows allow.
   bool enqueue_stats(hipStream_t stream);
+  // Automatic mode, before any statistics: false (report().reason =
+  // SplitNotFaster) when even the fewest moduli the tolerance allows, at the
+  // full-chip INT8 rate, cannot make the split form faster than fp64Seconds.
+  // plan() would reject every such product, so its statistics pass is skipped.
+  bool may_beat_fp64(double fp64Seconds);
   // False when the product falls back to the FP64 kernels (report().reason).
   bool plan(double fp64Seconds);
   // Device bytes of the residue planes and INT8 GEMM output (after plan()).

@@ -132,6 +132,17 @@ SplitPlan plan_split(const OperandStats& A, const OperandStats& B, double tolera
   return p;
 }
 
+int min_moduli(double tolerance) {
+  // scale_factor: ||X||^2 < comps k sum_i 4^(fl_i + 1), so S > 1/2 and
+  // P = ceil(log2(S / (tol / 2.02))) >= ceil(log2(1.01 / tol)); bitsNeeded is
+  // at least (P_A - 1) + (P_B - 1) + 2 (a scaled row's norm is >= 2^(P - 1))
+  const int P = std::max(1, static_cast<int>(std::ceil(std::log2(1.01 / tolerance))));
+  double bits = 0.0;
+  int n = 0;
+  while (n < kMaxModuli && bits < 2.0 * P) bits += std::log2(kModuliList[n++]);
+  return n;
+}
+
 void make_moduli(int count, ModuliSet& ms) {
   std::memset(&ms, 0, sizeof(ms));
   ms.count = count;

@@ -67,6 +67,11 @@ double fp64_bound(int64_t k, int comps);
 SplitPlan plan_split(const OperandStats& A, const OperandStats& B, double tolerance,
                      int maxModuli);
 
+// Fewest moduli plan_split can prescribe for a tolerance, whatever the data:
+// every row scale factor exceeds 1/2, so each operand needs at least
+// ceil(log2(1.01 / tolerance)) bits and the moduli must cover twice that.
+int min_moduli(double tolerance);
+
 void make_moduli(int count, ModuliSet& ms);
 void make_crt_constants(const ModuliSet& ms, CrtConstants& cc);
 

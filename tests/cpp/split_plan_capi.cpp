@@ -27,6 +27,8 @@ void plan_c(const double* maxA, const double* sumA, int rowsA, const double* max
   }
 }
 
+int min_moduli_c(double tol) { return spla_amd::min_moduli(tol); }
+
 // W: count x 3 limbs, M: 3 limbs, halfM: 3 limbs
 void crt_c(int count, unsigned long long* W, unsigned long long* M, unsigned long long* halfM,
            double* Md) {

@@ -39,6 +39,7 @@ def planner(tmp_path_factory):
     d, i = ctypes.c_void_p, ctypes.c_int
     lib.plan_c.argtypes = [d, d, i, d, d, i, ctypes.c_longlong, i, ctypes.c_double, i, d, d, d, d]
     lib.crt_c.argtypes = [i, d, d, d, d]
+    lib.min_moduli_c.argtypes = [ctypes.c_double]
     return lib
 
 
@@ -145,6 +146,44 @@ def test_plan_bound_and_moduli_cover_product(planner, cplx, spread):
     ib_obj = [[int(v) for v in row.real] for row in ib]
     biggest = max(abs(sum(x * y for x, y in zip(ra, rb))) for ra in ia_obj for rb in ib_obj)
     assert 2 * biggest < math.prod(ozaki.MODULI[:p["moduli"]])
+
+
+@pytest.mark.parametrize("comps", [1, 2])
+def test_min_moduli_is_a_lower_bound(planner, comps):
+    """The automatic mode skips the statistics pass when even min_moduli(tol)
+    moduli cannot beat FP64: no plan may ever need fewer. Random statistics,
+    and the ones that make the row scale factor smallest (every component at
+    the row maximum, maxima just below a power of two) or the rows sparsest
+    (one non-zero component, a power of two)."""
+    rng = np.random.default_rng(40 + comps)
+    seen = 0
+    for trial in range(400):
+        k = int(rng.integers(1, 400000))
+        tol = float(10.0 ** rng.uniform(-15.5, -6))
+        rows = int(rng.integers(1, 64))
+        kind = trial % 3
+        if kind == 0:
+            mA, sA = random_stats(rng, rows, k, comps, int(rng.integers(0, 40)))
+            mB, sB = random_stats(rng, rows + 3, k, comps, int(rng.integers(0, 40)))
+        elif kind == 1:  # dense rows: scale factor -> 1/2
+            e = rng.integers(-20, 20, size=rows)
+            mA = np.nextafter(np.exp2(e + 1.0), 0.0)
+            sA = mA ** 2 * k * comps
+            mB, sB = mA[::-1].copy(), sA[::-1].copy()
+        else:  # one non-zero component per row
+            mA = np.exp2(rng.integers(-20, 20, size=rows).astype(np.float64))
+            sA = mA ** 2
+            mB, sB = mA.copy(), sA.copy()
+        p = run_plan(planner, mA, sA, mB, sB, k, comps, tol)
+        if p["reason"] in (0, 4):  # planned, or only the moduli cap refused it
+            seen += 1
+            assert p["moduli"] >= planner.min_moduli_c(tol), (k, tol, kind)
+    assert seen > 300
+    # the bound itself: 2 ceil(log2(1.01 / tol)) bits of the moduli list
+    for tol in (1e-13, 1e-14, 3e-9):
+        bits = 2 * math.ceil(math.log2(1.01 / tol))
+        need = next(n for n in range(1, 21) if sum(math.log2(m) for m in ozaki.MODULI[:n]) >= bits)
+        assert planner.min_moduli_c(tol) == need
 
 
 def test_plan_fallbacks(planner):
