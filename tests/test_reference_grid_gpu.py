@@ -11,7 +11,7 @@ pgemm_ssb / pgemm_sbs over
 * sub-matrix offsets (0, 0) and (2, 3) clipped to the matrix as SPLA's tests do
   (rows max(1, m - 2), offset min(m - 1, 2), test_gemm_ssb.cpp:215-218);
 * alpha = 1, beta = 0, entries uniform in [0, 100), operands on the device and in
-  host memory;
+  host memory; here both the FP64 kernels and the split-precision form (forced);
 
 and compare with ScaLAPACK's p?gemm at |C - C_ref| <= 1e-6 per component. Here the
 reference is the double-double oracle of the same product; every entry outside
@@ -30,6 +30,14 @@ import spla_amd as spla
 from helpers import to_device
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["fp64", "split"])
+def form_ctx(request, device):
+    c = spla.Context(spla.ProcessingUnit.GPU)
+    if request.param == "split":
+        c.set_gemm_mode(spla.GemmMode.SPLIT_INT8)
+    return c, request.param
 
 SIZES = (1, 13, 32, 263)
 RANGES = ((0, 1), (50, 400))
@@ -65,6 +73,20 @@ def run(fn, args, where, device):
     return got
 
 
+def outcome(ctx, form, reduction):
+    """Split mode runs every product whose reduction has at least 50 terms in split
+    precision; shorter ones may fall back because the tolerance is capped at FP64's
+    own bound (reduction x 2^-53), which can need more than 53 bits per operand
+    (reasons 3 and 4). FP64 mode never splits."""
+    info = ctx.last_product_info()
+    if form != "split":
+        assert not info["split"]
+    elif reduction >= 50:
+        assert info["split"], info
+    elif reduction > 0 and not info["split"]:
+        assert info["reason"] in (3, 4), info
+
+
 def check(got, ref, C0, mask):
     assert np.array_equal(got[~mask], C0[~mask])  # outside the sub-matrix: untouched
     diff = got[mask] - ref[mask]
@@ -72,7 +94,8 @@ def check(got, ref, C0, mask):
 
 
 @pytest.mark.parametrize("tname", list(DTYPES))
-def test_pgemm_ssb_reference_grid(ctx, device, tname):
+def test_pgemm_ssb_reference_grid(form_ctx, device, tname):
+    ctx, form = form_ctx
     dtype = DTYPES[tname]
     rng = np.random.default_rng(4242)
     cases = 0
@@ -98,12 +121,14 @@ def test_pgemm_ssb_reference_grid(ctx, device, tname):
                 rows, cols, k, spla.Operation.CONJ_TRANSPOSE, 1.0, a, ld, b, ld, 0.0, c, m, rOff,
                 cOff, dist, ctx, dtype=dtype), dict(A=A, B=B, C=C0), where, device)
             check(got, ref, C0, mask)
+            outcome(ctx, form, k)
             cases += 1
     assert cases == 5 * 16 * 2 * 2 * 2
 
 
 @pytest.mark.parametrize("tname", list(DTYPES))
-def test_pgemm_sbs_reference_grid(ctx, device, tname):
+def test_pgemm_sbs_reference_grid(form_ctx, device, tname):
+    ctx, form = form_ctx
     dtype = DTYPES[tname]
     rng = np.random.default_rng(2424)
     cases = 0
@@ -129,17 +154,20 @@ def test_pgemm_sbs_reference_grid(ctx, device, tname):
                 m, cols, rows, 1.0, a, lda, b, k, rOff, cOff, dist, 0.0, c, ldc, ctx,
                 dtype=dtype), dict(A=A, B=B, C=C0), where, device)
             check(got, ref, C0, mask)
+            if m > 0:  # an empty product runs nothing
+                outcome(ctx, form, rows)
             cases += 1
     assert cases == 5 * 16 * 2 * 2 * 2
 
 
 @pytest.mark.parametrize("tname", list(DTYPES))
 @pytest.mark.parametrize("form", ["ssb", "sbs"])
-def test_large_reference_cases(ctx, device, tname, form):
+def test_large_reference_cases(form_ctx, device, tname, form):
     """LargeGemmSSB (m = 3000, n = 4000, reduction in [500, 6000],
     test_gemm_ssb.cpp:509-523) and LargeGemmSBS (local rows in [50, 400],
     n = 3000, k = 4000, test_gemm_sbs.cpp:514-525), offsets (2, 3), device
     operands; 512 sampled entries against the double-double oracle."""
+    ctx, mode = form_ctx
     dtype = DTYPES[tname]
     rng = np.random.default_rng(77 if form == "ssb" else 78)
     dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 128,
@@ -174,6 +202,7 @@ def test_large_reference_cases(ctx, device, tname, form):
     si = rng.integers(0, rows, size=512)
     sj = rng.integers(0, cols, size=512)
     ref = oracle.gemm_entries(opA, 0, red, 1.0, a, lda, b, ldb, 0.0, None, ldc, si, sj)
+    assert bool(ctx.last_product_info()["split"]) == (mode == "split")
     g = got[off + si + sj * ldc]
     diff = g - ref
     assert np.all(np.abs(diff.real) <= 1e-6) and np.all(np.abs(diff.imag) <= 1e-6), \
