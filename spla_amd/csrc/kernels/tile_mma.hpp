@@ -215,10 +215,12 @@ struct TileMma {
     TileStage<S, CPLX, LA, AR, BK, kUnchecked> stA;
     TileStage<S, CPLX, LB, BN, BK, kUnchecked> stB;
 
-    if constexpr (kF64) {
-      // FP64: the next 8-element sub-block's fragments are read from LDS while the
-      // current one's MFMAs run (two register sets; 2.6 % faster on the large
-      // complex tiles).
+    // FP64 with K-major A on the large tiles: the next 8-element sub-block's
+    // fragments are read from LDS while the current one's MFMAs run (two register
+    // sets; 2.6 % faster on the complex A^H B tiles). Measured slower elsewhere: 6 %
+    // on FP32, 0.7 % with MN-major A (A B), 0.5 % on the small ring-step tiles.
+    constexpr bool kPrefetch = kF64 && LA == KMajor && WM * WN >= 64 * 64;
+    if constexpr (kPrefetch) {
       static_assert((BK / 8) % 2 == 0, "sub-blocks alternate between two register sets");
       S a0[2][FM], a1[2][FM];
       V2 b[2][FN];
@@ -284,8 +286,7 @@ struct TileMma {
         if (more) read_frags(0, cur ^ 1, 0);  // the next k-block's first sub-block
       }
     } else {
-      // FP32: each sub-block's fragments are read right before its MFMAs (the
-      // prefetch measured 6 % slower here).
+      // each sub-block's fragments are read right before its MFMAs
       if (nk > 0) {
         stA.load(op.A, op.lda, arow0, op.m, kBegin, kEnd, tid);
         stB.load(op.B, op.ldb, col0, op.n, kBegin, kEnd, tid);
