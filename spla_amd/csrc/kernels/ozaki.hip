@@ -244,29 +244,29 @@ __global__ void __launch_bounds__(256) residue_kmajor_kernel(ResidueDesc d) {
   }
 }
 
-// Rows contiguous: a 64-row x 128-element tile is read along rows (a wave
-// reads 64 consecutive rows of one reduction index), every thread converts 16
-// consecutive reduction indices of its row into one 16-byte LDS word per
-// plane, and the tile leaves along the reduction index (128 contiguous bytes
-// per row and plane).
-constexpr int kResMnK = 128;  // reduction indices per block
+// Rows contiguous: each lane converts 16 consecutive reduction indices of one
+// row, which is exactly one 16-byte chunk of that row in every K-major plane, so
+// the transposition happens in registers (no LDS, no barrier). Eight lanes cover
+// the 128 reduction indices of a row, a wave eight rows: every load instruction
+// reads eight 128-byte row segments (one per reduction index) and every store
+// writes eight whole 128-byte plane rows. A block of 4 waves converts a
+// 32-row x 128-index tile.
+constexpr int kResMnK = 128;    // reduction indices per block
+constexpr int kResMnRows = 32;  // rows per block
 template <bool CPLX>
-__global__ void __launch_bounds__(512) residue_mnmajor_kernel(ResidueDesc d) {
+__global__ void __launch_bounds__(256) residue_mnmajor_kernel(ResidueDesc d) {
   constexpr int P = CPLX ? 3 : 1;
-  constexpr int kStride = kResMnK + 16;
-  __shared__ __attribute__((aligned(16))) int8_t lds[P][64][kStride];
-  const int tx = threadIdx.x & 63;
-  const int ty = threadIdx.x >> 6;  // 0..7: reduction indices 16 ty .. 16 ty + 15
-  const int row0 = blockIdx.y * 64;
-  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kResMnK + 16 * ty;
-  const int row = row0 + tx;
-  const int e = row < d.rows ? d.expo[row] : 0;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.y * kResMnRows + 8 * (threadIdx.x >> 6) + (lane >> 3);
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kResMnK + 16 * (lane & 7);
+  const bool live = row < d.rows;
+  const int e = live ? d.expo[row] : 0;
   Pieces re[16], im[16];  // re carries the +128 of the biased form in p0
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int64_t l = k0 + j;
     double vr = 0.0, vi = 0.0;
-    if (row < d.rows && l < d.k) {
+    if (live && l < d.k) {
       if (CPLX) {
         const double* p = d.X + (l * d.ld + row) * 2;
         vr = rint(ldexp(p[0], e));
@@ -279,8 +279,8 @@ __global__ void __launch_bounds__(512) residue_mnmajor_kernel(ResidueDesc d) {
     re[j].p0 += 128.0f;
     im[j] = pieces14(vi);
   }
-  const int wr = threadIdx.x >> 3;  // read-out: row wr, bytes 16 ws .. 16 ws + 15
-  const int ws = threadIdx.x & 7;
+  if (!live) return;
+  int8_t* base = d.R + static_cast<int64_t>(row) * d.kpad + k0;
   for (int t = 0; t < d.ms.count; ++t) {
     const ModConst kc = d.ms.kc[t];
     int4v w[P];
@@ -306,18 +306,8 @@ __global__ void __launch_bounds__(512) residue_mnmajor_kernel(ResidueDesc d) {
       }
     }
 #pragma unroll
-    for (int p = 0; p < P; ++p) *reinterpret_cast<int4v*>(&lds[p][tx][16 * ty]) = w[p];
-    __syncthreads();
-    if (row0 + wr < d.rows) {
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        int8_t* dst = d.R + static_cast<int64_t>(t * P + p) * d.plane +
-                      static_cast<int64_t>(row0 + wr) * d.kpad +
-                      static_cast<int64_t>(blockIdx.x) * kResMnK + 16 * ws;
-        *reinterpret_cast<int4v*>(dst) = *reinterpret_cast<const int4v*>(&lds[p][wr][16 * ws]);
-      }
-    }
-    __syncthreads();
+    for (int p = 0; p < P; ++p)
+      *reinterpret_cast<int4v*>(base + static_cast<int64_t>(t * P + p) * d.plane) = w[p];
   }
 }
 
@@ -724,11 +714,12 @@ hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipSt
     else
       hipLaunchKernelGGL(residue_kmajor_kernel<false>, grid, dim3(256), 0, stream, d);
   } else {
-    const dim3 grid(static_cast<unsigned>(d.kpad / kResMnK), (d.rows + 63) / 64);
+    const dim3 grid(static_cast<unsigned>(d.kpad / kResMnK),
+                    (d.rows + kResMnRows - 1) / kResMnRows);
     if (complex)
-      hipLaunchKernelGGL(residue_mnmajor_kernel<true>, grid, dim3(512), 0, stream, d);
+      hipLaunchKernelGGL(residue_mnmajor_kernel<true>, grid, dim3(256), 0, stream, d);
     else
-      hipLaunchKernelGGL(residue_mnmajor_kernel<false>, grid, dim3(512), 0, stream, d);
+      hipLaunchKernelGGL(residue_mnmajor_kernel<false>, grid, dim3(256), 0, stream, d);
   }
   return hipGetLastError();
 }
