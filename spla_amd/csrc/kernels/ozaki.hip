@@ -488,8 +488,16 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
 #pragma unroll
       for (int i = 0; i < kMfma; ++i) {
         __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (ks + 1 < kSub && i < kReads) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        if (i % 2 == 0 && i / 2 < kLoads) {
+        // with room for it, the LDS reads go two per MFMA right at the start (the
+        // earlier they are issued the less the next sub-step waits), the global
+        // loads after every other MFMA that follows
+        constexpr bool kEarly = kReads == 8 && kMfma == 16;
+        const bool early = kEarly && i < kReads / 2;
+        const bool loadSlot = kEarly ? (i >= kReads / 2 && i % 2 == 0 && (i - kReads / 2) / 2 < kLoads)
+                                     : (i % 2 == 0 && i / 2 < kLoads);
+        if (ks + 1 < kSub && early) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+        if (ks + 1 < kSub && !kEarly && i < kReads) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (loadSlot) {
           __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
           __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
         }
