@@ -150,6 +150,19 @@ bool SplitProduct<T>::plan(double fp64Seconds) {
   }
   splits_ = (kpad_ + kMaxSplitK - 1) / kMaxSplitK;
   while (tiles * G_ * splits_ < 2 * numCUs && kpad_ / (splits_ + 1) >= 2048) ++splits_;
+  if (splits_ > 1) {
+    // One workgroup per CU: the GEMM takes ceil(blocks / CUs) waves of kpad / splits
+    // each. Already summing INT32 splits, up to three more splits are taken when they
+    // fill the last wave better (e.g. 15 products x 64 tiles: 2 splits leave the 8th
+    // wave half empty, 4 splits make exactly 15 full waves).
+    auto cost = [&](int64_t sp) {
+      return std::ceil(static_cast<double>(tiles * G_ * sp) / numCUs) / static_cast<double>(sp);
+    };
+    int64_t best = splits_;
+    for (int64_t sp = splits_ + 1; sp <= splits_ + 3 && kpad_ / sp >= 2048; ++sp)
+      if (cost(sp) < 0.97 * cost(best)) best = sp;
+    splits_ = best;
+  }
   kSplit_ = round_up((kpad_ + splits_ - 1) / splits_, kKpadAlign);
   splits_ = (kpad_ + kSplit_ - 1) / kSplit_;
   if (ctx_.gemm_mode() == GemmModeAuto) {
