@@ -609,6 +609,26 @@ __device__ __forceinline__ void product_residues(const CrtDesc& d, int t, int64_
   }
 }
 
+// The same from preloaded words of biased residues r + 128 (four consecutive
+// rows per word, one word per complex sub-product).
+template <bool CPLX>
+__device__ __forceinline__ void residues_from_words(const uint32_t* w, double m, double inv,
+                                                    double (&re)[4], double (&im)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    double T[3];
+#pragma unroll
+    for (int p = 0; p < (CPLX ? 3 : 1); ++p)
+      T[p] = static_cast<double>(static_cast<int>((w[p] >> (8 * u)) & 0xffu) - 128);
+    if (CPLX) {
+      re[u] = crt::sym_residue(T[0] - T[1], m, inv);
+      im[u] = crt::sym_residue(T[2] - T[0] - T[1], m, inv);
+    } else {
+      re[u] = crt::sym_residue(T[0], m, inv);
+    }
+  }
+}
+
 template <bool CPLX>
 __device__ __forceinline__ void update_c(const CrtDesc& d, int64_t i, int64_t j, double re,
                                          double im) {
@@ -655,9 +675,7 @@ __global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int l = 0; l < L; ++l) Sre[u][l] = Sim[u][l] = 0.0;
-  auto modulus = [&](int t) {
-    double re[4], im[4];
-    product_residues<CPLX, IN8>(d, t, off, re, im);
+  auto accumulate = [&](int t, const double (&re)[4], const double (&im)[4]) {
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const double w = cc.Wl[t][l];
@@ -668,7 +686,36 @@ __global__ void __launch_bounds__(256) crt_kernel(CrtDesc d) {
       }
     }
   };
-  for (int t = 0; t < d.ms.count; ++t) modulus(t);
+  if constexpr (IN8) {
+    // every residue word is loaded before any is used, so the loads of all
+    // moduli are in flight together
+    constexpr int P = CPLX ? 3 : 1;
+    uint32_t words[kMaxModuli][P];
+#pragma unroll
+    for (int t = 0; t < kMaxModuli; ++t) {
+      if (t < d.ms.count) {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+          words[t][p] = *reinterpret_cast<const uint32_t*>(
+              static_cast<const uint8_t*>(d.W) +
+              (static_cast<int64_t>(t) * d.products + p) * d.plane + off);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kMaxModuli; ++t) {
+      if (t < d.ms.count) {
+        double re[4], im[4];
+        residues_from_words<CPLX>(words[t], d.ms.m[t], d.ms.inv[t], re, im);
+        accumulate(t, re, im);
+      }
+    }
+  } else {
+    for (int t = 0; t < d.ms.count; ++t) {
+      double re[4], im[4];
+      product_residues<CPLX, false>(d, t, off, re, im);
+      accumulate(t, re, im);
+    }
+  }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int64_t i = i0 + u;
