@@ -51,7 +51,7 @@ INT8_PEAK_TOPS = 5000.0
 # --copy-calibration ...), each scaled by what the counter reads for a 1 GiB device
 # copy (reads x 2.0, writes x 1.0). They change only when the kernels or the split
 # plan (number of moduli) change.
-HBM_TRAFFIC_BYTES = {"split": 51.66e9 + 1.41e9, "plain": 118.46e9 + 0.07e9}
+HBM_TRAFFIC_BYTES = {"split": 51.66e9 + 1.41e9, "plain": 116.67e9 + 0.07e9}
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
 
