@@ -190,6 +190,14 @@ def cpu_baseline(m, n, k_full, target_seconds=12.0):
     k2 = int(min(k_full, max(k1, k1 * target_seconds / max(t1, 1e-3))) // 1024 * 1024)
     t2 = run(k2) if k2 > k1 else t1
     rate = 8.0 * m * n * k2 / t2 / 1e12
+    one = ""
+    if threads and threads > 1:
+        oracle.lib().cpu_path_set_threads(1)
+        try:
+            t1s = run(k1)
+        finally:
+            oracle.lib().cpu_path_set_threads(threads)
+        one = f"; 1 thread at k={k1}: {8.0 * m * n * k1 / t1s / 1e12:.4f} TFLOP/s"
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
@@ -201,12 +209,13 @@ def cpu_baseline(m, n, k_full, target_seconds=12.0):
         "kind": "port",
         "sample": (f"{info['vendor']} cblas_zgemm('C','N') m=n={m}, k={k2} (of {k_full}); "
                    f"{t2:.2f} s; extrapolated full-k step {t2 * k_full / k2:.1f} s; "
+                   f"all threads at k={k1}: {8.0 * m * n * k1 / t1 / 1e12:.4f} TFLOP/s{one}; "
                    f"host cores visible {cores}"),
     }
 
 
 WORKLOADS = ("ssb2048", "sbs100k", "ssb_real_spread", "batch512", "ssb2048_host",
-             "ssb2048_c64")
+             "ssb2048_c64", "ssb_small_mn")
 
 
 def _timed(fn, steps, warmup, stream):
@@ -429,6 +438,51 @@ def side_workload(args) -> int:
         out["config"] = {"workload": "spla_pcgemm_ssb A^H B complex float m=n=2048 k=200000, "
                                      "device-resident, FP32 accumulation"}
         out["dtype"] = "c64"
+
+    elif wl == "ssb_small_mn":
+        # HBM-bound shapes: complex intensity m n / (2 (m + n)) FLOP/B is below the FP64
+        # ridge 78.6 / 8 = 9.8 FLOP/B for m = n <= 39; plain FP64 kernels (split-K)
+        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
+                                                                 256, 256)
+        ctx.set_gemm_mode(spla.GemmMode.FP64)
+        for (m, n, k) in ((16, 16, 8_000_000), (32, 32, 4_000_000)):
+            dA = uniform_device(SEED, 1, k * m, True, dev)
+            dB = uniform_device(SEED, 2, k * n, True, dev)
+            dC = uniform_device(SEED, 3, m * n, True, dev)
+            C0 = dC.clone()
+            flops = 8.0 * m * n * k
+            hbm = 16.0 * k * (m + n) + 2 * 16.0 * m * n  # A, B once; C read (beta != 0) and written
+            run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, dA, k, dB,
+                                         k, BETA, dC, m, 0, 0, dist, ctx)
+            dC.copy_(C0)
+            run()
+            rows = np.random.default_rng(5).integers(0, m, 8)
+            cols = np.random.default_rng(6).integers(0, n, 8)
+            a = dA.view(m, k)[torch.from_numpy(rows).to(dev)].cpu().numpy()
+            b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy()
+            re = _accurate_dot(np.concatenate([a.real, a.imag], -1),
+                               np.concatenate([b.real, b.imag], -1))
+            im = _accurate_dot(np.concatenate([a.real, -a.imag], -1),
+                               np.concatenate([b.imag, b.real], -1))
+            c0 = C0.view(n, m).cpu().numpy()[cols, rows]
+            ref = ALPHA * (re + 1j * im) + BETA * c0
+            got = dC.view(n, m).cpu().numpy()[cols, rows]
+            nAB = (torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item())
+            err = float(np.sqrt(np.mean(np.abs((got - ref) / nAB) ** 2) * m * n))
+            wall, ev = _timed(run, steps, warmup, stream)
+            gbs = hbm / (wall / 1e3) / 1e9
+            key = f"m=n={m} k={k}"
+            out["forms"][key] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
+                                 "device_ms_per_step": round(ev, 3),
+                                 "hbm_gb_per_s": round(gbs, 1),
+                                 "fraction_of_hbm_peak": round(gbs / 8000.0, 4),
+                                 "intensity_flop_per_byte": round(flops / hbm, 2),
+                                 "normwise_error": err, "phase_ms": _phases(ctx, run)}
+            log(f"ssb_small_mn {key}: {out['forms'][key]}")
+            del dA, dB, dC, C0
+        out["config"] = {"workload": "spla_pzgemm_ssb A^H B complex, m = n = 16 (k = 8e6) and "
+                                     "32 (k = 4e6), device-resident, plain FP64, HBM-bound"}
+        out["dtype"] = "f64"
 
     print(json.dumps(out), flush=True)
     return 0

@@ -167,6 +167,53 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmDesc d, int tile
   store_result<S, CPLX>(d, i, j, re, im);
 }
 
+// Many splits over few elements (small m, n, long reduction): 16 elements per
+// block, 16 split groups per element. Group g sums splits g, g + 16, g + 32, ...
+// in order and the 16 group sums are added in group order, so the result is
+// bitwise reproducible; every split is read by 16 consecutive elements at once.
+constexpr int kWideElems = 16;
+constexpr int kWideGroups = 16;
+template <typename S, bool CPLX>
+__global__ void __launch_bounds__(256) splitk_reduce_wide_kernel(GemmDesc d, int tileRows,
+                                                                 int tileCols) {
+  __shared__ S part[kWideGroups][kWideElems][2];
+  const int e = threadIdx.x % kWideElems;
+  const int g = threadIdx.x / kWideElems;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kWideElems + e;
+  const int64_t total = static_cast<int64_t>(d.m) * d.n;
+  bool live = idx < total;
+  const int i = live ? static_cast<int>(idx % d.m) : 0;
+  const int j = live ? static_cast<int>(idx / d.m) : 0;
+  if (live && d.fillMode != FillFull) {
+    const int64_t r0 = (i / tileRows) * static_cast<int64_t>(tileRows);
+    const int64_t c0 = (j / tileCols) * static_cast<int64_t>(tileCols);
+    const int64_t nr = min<int64_t>(tileRows, d.m - r0);
+    const int64_t nc = min<int64_t>(tileCols, d.n - c0);
+    live = tile_active(d, r0, c0, nr, nc);
+  }
+  S re = S(0), im = S(0);
+  if (live) {
+    const int64_t stride = static_cast<int64_t>(d.n) * d.Mp;
+    const S* w = static_cast<const S*>(d.work) + static_cast<int64_t>(j) * d.Mp + (CPLX ? 2 * i : i);
+    for (int sp = g; sp < d.splits; sp += kWideGroups) {
+      re += w[sp * stride];
+      if (CPLX) im += w[sp * stride + 1];
+    }
+  }
+  part[g][e][0] = re;
+  part[g][e][1] = im;
+  __syncthreads();
+  if (g == 0 && live) {
+    S sr = part[0][e][0], si = part[0][e][1];
+#pragma unroll
+    for (int q = 1; q < kWideGroups; ++q) {
+      sr += part[q][e][0];
+      si += part[q][e][1];
+    }
+    store_result<S, CPLX>(d, i, j, sr, si);
+  }
+}
+
 // C <- beta * C (k == 0 or alpha == 0); beta == 0 writes zeros without reading C.
 template <typename S, bool CPLX>
 __global__ void __launch_bounds__(256) scale_kernel(GemmDesc d) {
@@ -259,6 +306,16 @@ hipError_t launch_splitk_reduce_s(const GemmDesc& d, bool complex, int cfg, hipS
   const TileShape ts = tile_shape(cfg);
   const int tileRows = complex ? ts.bm / 2 : ts.bm;
   const int64_t total = static_cast<int64_t>(d.m) * d.n;
+  if (d.splits > 64) {  // only the whole-device plans of small m, n split this far
+    const dim3 wide(static_cast<unsigned>((total + kWideElems - 1) / kWideElems));
+    if (complex)
+      hipLaunchKernelGGL((splitk_reduce_wide_kernel<S, true>), wide, dim3(256), 0, stream, d,
+                         tileRows, ts.bn);
+    else
+      hipLaunchKernelGGL((splitk_reduce_wide_kernel<S, false>), wide, dim3(256), 0, stream, d,
+                         tileRows, ts.bn);
+    return hipGetLastError();
+  }
   const dim3 grid(static_cast<unsigned>((total + 255) / 256));
   if (complex)
     hipLaunchKernelGGL((splitk_reduce_kernel<S, true>), grid, dim3(256), 0, stream, d, tileRows,

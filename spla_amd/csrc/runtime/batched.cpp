@@ -105,7 +105,7 @@ void execute_split_batch(spla::ContextInternal& ctx, const std::vector<BlockSpec
     it.split = !alphaZero &&
                (!auto_ || it.p->may_beat_fp64(fp64_estimate_seconds(
                               ctx, !std::is_same<T, double>::value, opA, SPLA_OP_NONE, b.m, b.n,
-                              b.k))) &&
+                              b.k, /*wholeChip=*/false))) &&
                it.p->enqueue_stats(lanes[it.lane]);
     items.push_back(std::move(it));
   }
@@ -116,7 +116,8 @@ void execute_split_batch(spla::ContextInternal& ctx, const std::vector<BlockSpec
     if (!it.split) continue;
     const BlockSpec& b = *it.spec;
     const double fp64 = ctx.gemm_mode() == GemmModeAuto
-                            ? fp64_estimate_seconds(ctx, cplx, opA, SPLA_OP_NONE, b.m, b.n, b.k)
+                            ? fp64_estimate_seconds(ctx, cplx, opA, SPLA_OP_NONE, b.m, b.n, b.k,
+                                                    /*wholeChip=*/false)
                             : 0.0;
     it.split = it.p->plan(fp64);
     if (it.split) laneBytes[it.lane] = std::max(laneBytes[it.lane], it.p->workspace_bytes());

@@ -132,7 +132,7 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
                             T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                             T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
                             hipStream_t stream, std::vector<Block>& keep, Block* reuse = nullptr,
-                            bool trySplit = true) {
+                            bool trySplit = true, bool wholeChip = true) {
   using S = Scalar<T>;
   constexpr bool cplx = S::kComplex;
   constexpr bool f32 = S::kF32;
@@ -170,7 +170,7 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
 
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
   const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
-  const GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus());
+  const GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus(), wholeChip);
   if constexpr (!f32) {
     if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
         enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
@@ -471,15 +471,16 @@ void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t 
                        int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
                        hipStream_t stream, std::vector<Block>& keep, Block* reuse,
                        bool trySplit) {
+  // the block products of a batch share the device with the other streams
   enqueue_device_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
-                            stream, keep, reuse, trySplit);
+                            stream, keep, reuse, trySplit, /*wholeChip=*/false);
 }
 
 double fp64_estimate_seconds(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
-                             SplaOperation opB, int64_t m, int64_t n, int64_t k) {
+                             SplaOperation opB, int64_t m, int64_t n, int64_t k, bool wholeChip) {
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
   const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
-  return ctx.tuner().plan(complex, la, lb, m, n, k, ctx.num_cus()).estSeconds;
+  return ctx.tuner().plan(complex, la, lb, m, n, k, ctx.num_cus(), wholeChip).estSeconds;
 }
 
 template void enqueue_on_device<double>(SplaOperation, SplaOperation, int64_t, int64_t, int64_t,

@@ -53,9 +53,11 @@ void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t 
                        bool trySplit = true);
 
 // Estimated seconds of the FP64 kernels for one product (the tuner's model),
-// which the automatic mode compares with the split form.
+// which the automatic mode compares with the split form; wholeChip = false for
+// the block products of a batch, which share the device.
 double fp64_estimate_seconds(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
-                             SplaOperation opB, int64_t m, int64_t n, int64_t k);
+                             SplaOperation opB, int64_t m, int64_t n, int64_t k,
+                             bool wholeChip = true);
 
 // SPLA's check_gemm_param (src/util/check_gemm_param.hpp:36-50): throws
 // InvalidParameterError / InvalidPointerError.

@@ -44,7 +44,7 @@ int env_tile() {
 }  // namespace
 
 GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, int forceCfg,
-                   int forceSplits, const double* effByCfg) {
+                   int forceSplits, const double* effByCfg, bool wholeChip) {
   const int64_t Mp = complex ? 2 * m : m;
   const int64_t Kp = complex ? 2 * k : k;
   const int cands[] = {TileL, TileM, TileXL};
@@ -61,8 +61,17 @@ GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, in
     const int bpc = blocks_per_cu(cfg);
     const int64_t slots = static_cast<int64_t>(numCUs) * bpc;
     const double eff = effByCfg ? effByCfg[cfg] : 0.8;
-    // split factors: every split keeps at least 8 k-blocks
-    const int64_t maxSplits = std::max<int64_t>(1, std::min<int64_t>(kBlocks / 8, 64));
+    // split factors: every split keeps at least 8 k-blocks; up to 64, or, for a
+    // product that has the device to itself, as many as fill the resident workgroup
+    // slots when there are few tiles (small m, n and a long reduction: HBM-bound,
+    // every CU must stream its share of A and B). Batch blocks share the device
+    // with the blocks on the other streams and keep the cap of 64.
+    const int64_t fillSplits = wholeChip ? slots / std::max<int64_t>(1, tiles) : 0;
+    const int64_t maxSplits = std::max<int64_t>(
+        1, std::min<int64_t>(kBlocks / 8, std::max<int64_t>(64, fillSplits)));
+    // A and B are read once whatever the split; with fewer workgroups than CUs only
+    // that share of the HBM bandwidth is drawn
+    const double bytes = static_cast<double>(m + n) * static_cast<double>(k) * (complex ? 16.0 : 8.0);
     for (int64_t s = 1; s <= maxSplits; s = (s < 4 ? s + 1 : s * 3 / 2)) {
       if (forceSplits > 0 && s != std::min<int64_t>(forceSplits, maxSplits)) continue;
       const int64_t kbPerSplit = (kBlocks + s - 1) / s;
@@ -72,7 +81,9 @@ GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, in
       const int64_t waves = (blocks + slots - 1) / slots;
       const double blockFlops = 2.0 * ts.bm * ts.bn * kbPerSplit * ts.bk;
       const double waveSeconds = blockFlops * bpc / (kPeakPerCU * eff);
-      double t = waves * waveSeconds + kLaunchSeconds;
+      const double active = std::min(1.0, static_cast<double>(blocks) / numCUs);
+      const double tMem = wholeChip ? bytes / (kHbmBytesPerSecond * active) : 0.0;
+      double t = std::max(waves * waveSeconds, tMem) + kLaunchSeconds;
       if (splits > 1)
         t += (splits + 2) * 8.0 * Mp * n / kHbmBytesPerSecond + kLaunchSeconds;
       if (t < best.estSeconds * 0.999) {
@@ -108,10 +119,10 @@ void Tuner::record_efficiency(int cfg, double eff) {
 }
 
 GemmPlan Tuner::plan(bool complex, int layoutA, int layoutB, int64_t m, int64_t n, int64_t k,
-                     int numCUs) {
+                     int numCUs, bool wholeChip) {
   const int forceCfg = env_tile();
   const int forceSplits = env_int("SPLA_AMD_SPLITS", 0);
-  const Key key{complex, layoutA, layoutB, m, n, k, numCUs};
+  const Key key{complex, layoutA, layoutB, m, n, k, numCUs, wholeChip};
   std::lock_guard<std::mutex> guard(lock_);
   if (forceCfg < 0 && forceSplits <= 0) {
     auto it = cache_.find(key);
@@ -119,7 +130,7 @@ GemmPlan Tuner::plan(bool complex, int layoutA, int layoutB, int64_t m, int64_t 
   }
   double eff[NumTileConfigs];
   for (int c = 0; c < NumTileConfigs; ++c) eff[c] = efficiency(c);
-  GemmPlan p = plan_gemm(complex, m, n, k, numCUs, forceCfg, forceSplits, eff);
+  GemmPlan p = plan_gemm(complex, m, n, k, numCUs, forceCfg, forceSplits, eff, wholeChip);
   if (forceCfg < 0 && forceSplits <= 0) cache_[key] = p;
   return p;
 }

@@ -27,15 +27,17 @@ struct GemmPlan {
 class Tuner {
 public:
   // complex: complex operands; m, n, k in elements; numCUs of the device.
+  // wholeChip: the product has the device to itself (a single call); false for the
+  // block products of a batch, which run side by side on several streams.
   GemmPlan plan(bool complex, int layoutA, int layoutB, int64_t m, int64_t n, int64_t k,
-                int numCUs);
+                int numCUs, bool wholeChip = true);
 
   // Sustained fraction of the per-CU FP64 matrix rate assumed for a tile config.
   double efficiency(int cfg) const;
   void record_efficiency(int cfg, double eff);
 
 private:
-  using Key = std::tuple<bool, int, int, int64_t, int64_t, int64_t, int>;
+  using Key = std::tuple<bool, int, int, int64_t, int64_t, int64_t, int, bool>;
   std::mutex lock_;
   std::map<Key, GemmPlan> cache_;
   std::map<int, double> eff_;
@@ -43,6 +45,6 @@ private:
 
 // Pure cost model (no state), exposed for tests.
 GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, int forceCfg,
-                   int forceSplits, const double* effByCfg);
+                   int forceSplits, const double* effByCfg, bool wholeChip = true);
 
 }  // namespace spla_amd

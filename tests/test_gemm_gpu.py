@@ -122,6 +122,30 @@ def test_split_k_shapes(ctx, device, dtype, shape):
     assert err <= TOL, err
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
+@pytest.mark.parametrize("shape", [(16, 16, 400000), (1, 1, 1000000), (33, 7, 300000)])
+def test_small_mn_long_reduction(ctx, device, dtype, shape):
+    """HBM-bound shapes: hundreds of split-K partial sums, reduced by the wide
+    fixed-order reduction; accurate and bitwise identical from run to run."""
+    import torch
+
+    m, n, k = shape
+    A, B, C0, got, ref, sa, sb, ldc = run_case(ctx, device, dtype, "C", "N", m, n, k,
+                                               alpha=1.5, beta=0.5)
+    err = rel_error(to_matrix(got, m, n, ldc), to_matrix(ref, m, n, ldc), to_matrix(A, *sa),
+                    to_matrix(B, *sb))
+    assert err <= TOL, err
+    dA, dB = to_device(A, device), to_device(B, device)
+    outs = []
+    for _ in range(2):
+        dC = to_device(C0, device)
+        spla.gemm(spla.Operation.CONJ_TRANSPOSE, spla.Operation.NONE, m, n, k, 1.5, dA, sa[2], dB,
+                  sb[2], 0.5, dC, ldc, ctx)
+        torch.cuda.synchronize()
+        outs.append(dC.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_run_to_run_bitwise_deterministic(ctx, device):
     m, n, k = 96, 80, 30000  # split-K active
     A = make_buffer(9, 1, k, m, k, np.complex128)
