@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "kernels/gemm_desc.hpp"
 
@@ -69,11 +70,20 @@ GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, in
     const int64_t fillSplits = wholeChip ? slots / std::max<int64_t>(1, tiles) : 0;
     const int64_t maxSplits = std::max<int64_t>(
         1, std::min<int64_t>(kBlocks / 8, std::max<int64_t>(64, fillSplits)));
-    // A and B are read once whatever the split; with fewer workgroups than CUs only
-    // that share of the HBM bandwidth is drawn
+    // A and B are read once whatever the split; with fewer workgroups than resident
+    // slots only that share of the HBM bandwidth is drawn
     const double bytes = static_cast<double>(m + n) * static_cast<double>(k) * (complex ? 16.0 : 8.0);
-    for (int64_t s = 1; s <= maxSplits; s = (s < 4 ? s + 1 : s * 3 / 2)) {
-      if (forceSplits > 0 && s != std::min<int64_t>(forceSplits, maxSplits)) continue;
+    // candidates 1, 2, 3, 4, 6, 9, ... up to maxSplits and the count that fills every
+    // resident slot once, or exactly the forced count
+    const int64_t forced = forceSplits > 0 ? std::min<int64_t>(forceSplits, maxSplits) : 0;
+    std::vector<int64_t> cand;
+    if (forced > 0) {
+      cand.push_back(forced);
+    } else {
+      for (int64_t s = 1; s <= maxSplits; s = (s < 4 ? s + 1 : s * 3 / 2)) cand.push_back(s);
+      if (fillSplits > 0 && fillSplits <= maxSplits) cand.push_back(fillSplits);
+    }
+    for (const int64_t s : cand) {
       const int64_t kbPerSplit = (kBlocks + s - 1) / s;
       const int64_t splits = (kBlocks + kbPerSplit - 1) / kbPerSplit;  // no empty split
       const int64_t blocks = tiles * splits;
@@ -81,7 +91,8 @@ GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, in
       const int64_t waves = (blocks + slots - 1) / slots;
       const double blockFlops = 2.0 * ts.bm * ts.bn * kbPerSplit * ts.bk;
       const double waveSeconds = blockFlops * bpc / (kPeakPerCU * eff);
-      const double active = std::min(1.0, static_cast<double>(blocks) / numCUs);
+      // HBM throughput grows with the resident workgroups (their loads in flight)
+      const double active = std::min(1.0, static_cast<double>(blocks) / slots);
       const double tMem = wholeChip ? bytes / (kHbmBytesPerSecond * active) : 0.0;
       double t = std::max(waves * waveSeconds, tMem) + kLaunchSeconds;
       if (splits > 1)
@@ -95,6 +106,15 @@ GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, in
         best.estSeconds = t;
       }
     }
+  }
+  if (best.kSplit <= 0) {
+    // no candidate passed (e.g. a forced tile whose grid would not fit): one split
+    best.cfg = forceCfg >= 0 ? forceCfg : TileL;
+    const TileShape ts = tile_shape(best.cfg);
+    best.splits = 1;
+    best.kSplit = std::max<int64_t>(1, (Kp + ts.bk - 1) / ts.bk) * ts.bk;
+    best.tilesM = static_cast<int>((Mp + ts.bm - 1) / ts.bm);
+    best.tilesN = static_cast<int>((n + ts.bn - 1) / ts.bn);
   }
   return best;
 }
