@@ -285,6 +285,73 @@ struct TileMma {
         __syncthreads();
         if (more) read_frags(0, cur ^ 1, 0);  // the next k-block's first sub-block
       }
+    } else if constexpr (kF64 && WM * WN < 64 * 64) {
+      // Small FP64 tiles (small m, n; HBM-bound with a long reduction): the global
+      // loads run two k-blocks ahead through two register sets, so each workgroup
+      // keeps twice the bytes in flight; fragments are read just in time.
+      TileStage<S, CPLX, LA, AR, BK, kUnchecked> stA2;
+      TileStage<S, CPLX, LB, BN, BK, kUnchecked> stB2;
+      if (nk > 0) {
+        stA.load(op.A, op.lda, arow0, op.m, kBegin, kEnd, tid);
+        stB.load(op.B, op.ldb, col0, op.n, kBegin, kEnd, tid);
+        stA.store(sm.lA[0], tid);
+        stB.store(sm.lB[0], tid);
+      }
+      if (nk > 1) {
+        stA2.load(op.A, op.lda, arow0, op.m, kBegin + BK, kEnd, tid);
+        stB2.load(op.B, op.ldb, col0, op.n, kBegin + BK, kEnd, tid);
+      }
+      __syncthreads();
+      // one k-block: (la, lb) receive k-block it + 2, (sa, sb) hold it + 1
+      auto kblock = [&](int it, auto& la, auto& lb, const auto& sa, const auto& sb) {
+        const int cur = it & 1;
+        if (it + 2 < nk) {
+          const int64_t q0 = kBegin + static_cast<int64_t>(it + 2) * BK;
+          la.load(op.A, op.lda, arow0, op.m, q0, kEnd, tid);
+          lb.load(op.B, op.ldb, col0, op.n, q0, kEnd, tid);
+        }
+#pragma unroll
+        for (int kb8 = 0; kb8 < BK / 8; ++kb8) {
+          S a0[FM], a1[FM];
+          V2 b[FN];
+#pragma unroll
+          for (int f = 0; f < FM; ++f) {
+            const int sr = CPLX ? (wm * WM) / 2 + 8 * f + cplx_row(l16) : wm * WM + 16 * f + l16;
+            const V2 v = *reinterpret_cast<const V2*>(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
+            if (CPLX) {
+              const S x = swapPair ? v.y : v.x;
+              const S y = swapPair ? v.x : v.y;
+              a0[f] = flip_sign(x, sgn0);
+              a1[f] = flip_sign(y, sgn1);
+            } else {
+              a0[f] = v.x;
+              a1[f] = v.y;
+            }
+          }
+#pragma unroll
+          for (int g = 0; g < FN; ++g) {
+            const int j = wn * WN + 16 * g + l16;
+            b[g] = *reinterpret_cast<const V2*>(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
+          }
+#pragma unroll
+          for (int f = 0; f < FM; ++f)
+#pragma unroll
+            for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a0[f], b[g].x, acc[f][g]);
+#pragma unroll
+          for (int f = 0; f < FM; ++f)
+#pragma unroll
+            for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a1[f], b[g].y, acc[f][g]);
+        }
+        if (it + 1 < nk) {
+          sa.store(sm.lA[cur ^ 1], tid);
+          sb.store(sm.lB[cur ^ 1], tid);
+        }
+        __syncthreads();
+      };
+      for (int it = 0; it < nk; it += 2) {
+        kblock(it, stA, stB, stA2, stB2);
+        if (it + 1 < nk) kblock(it + 1, stA2, stB2, stA, stB);
+      }
     } else {
       // each sub-block's fragments are read right before its MFMAs
       if (nk > 0) {
