@@ -78,3 +78,25 @@ def test_hbm_bound_shapes_fill_the_device(tuner):
     assert q["splits"] <= 64  # batch blocks share the device
     h = plan(tuner, True, 2048, 2048, 200000)
     assert h["splits"] == 1 and h["tilesM"] * h["tilesN"] == 512  # 128 x 128 embedded tiles
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_gemm_grid_reaches_every_kblock_count_of_small_tiles(tuner, cplx):
+    """The GEMM reference grid (m, n, k in 1, 13, 32, 263; every operation pair) runs the
+    64 x 64 tile, whose FP64 loads go two k-blocks ahead, with splits of one and of two
+    k-blocks (the loop's prologue edges) and with odd and even longer loops."""
+    sizes = (1, 13, 32, 263)
+    counts = set()
+    for m in sizes:
+        for n in sizes:
+            for k in sizes:
+                p = plan(tuner, cplx, m, n, k)
+                if p["cfg"] != 1:  # TileM
+                    continue
+                kp = -(-(2 if cplx else 1) * k // 16) * 16
+                for s in range(p["splits"]):
+                    left = kp - s * p["kSplit"]
+                    if left > 0:
+                        counts.add(min(p["kSplit"], left) // 16)
+    assert {1, 2} <= counts
+    assert any(c >= 3 and c % 2 for c in counts) and any(c >= 4 and c % 2 == 0 for c in counts)
