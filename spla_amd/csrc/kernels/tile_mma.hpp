@@ -285,13 +285,12 @@ struct TileMma {
         __syncthreads();
         if (more) read_frags(0, cur ^ 1, 0);  // the next k-block's first sub-block
       }
-    } else if constexpr (kF64 && WM * WN < 64 * 64 &&
-                         (CPLX || LA != MNMajor || LB != MNMajor)) {
-      // Small FP64 tiles (small m, n; HBM-bound with a long reduction): the global
-      // loads run two k-blocks ahead through two register sets, so each workgroup
-      // keeps twice the bytes in flight; fragments are read just in time. The real
-      // MN-major x MN-major variant stays one ahead: the second set takes it past
-      // 128 VGPRs, one resident workgroup fewer per CU, and it ran 7 % slower.
+    } else if constexpr (WM * WN < 64 * 64 && (CPLX || LA != MNMajor || LB != MNMajor)) {
+      // Small tiles (small m, n; HBM-bound with a long reduction): the global loads
+      // run two k-blocks ahead through two register sets, so each workgroup keeps
+      // twice the bytes in flight; fragments are read just in time. The real
+      // MN-major x MN-major variant stays one ahead: in FP64 the second set takes it
+      // past 128 VGPRs, one resident workgroup fewer per CU, and it ran 7 % slower.
       TileStage<S, CPLX, LA, AR, BK, kUnchecked> stA2;
       TileStage<S, CPLX, LB, BN, BK, kUnchecked> stB2;
       if (nk > 0) {
