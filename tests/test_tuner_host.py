@@ -83,7 +83,7 @@ def test_hbm_bound_shapes_fill_the_device(tuner):
 @pytest.mark.parametrize("cplx", [False, True])
 def test_gemm_grid_reaches_every_kblock_count_of_small_tiles(tuner, cplx):
     """The GEMM reference grid (m, n, k in 1, 13, 32, 263; every operation pair) runs the
-    64 x 64 tile, whose FP64 loads go two k-blocks ahead, with splits of one and of two
+    64 x 64 tile, whose loads go two k-blocks ahead, with splits of one and of two
     k-blocks (the loop's prologue edges) and with odd and even longer loops."""
     sizes = (1, 13, 32, 263)
     counts = set()
