@@ -56,11 +56,23 @@ SPLA_EXPORT SplaError spla_amd_ctx_set_gemm_mode(SplaContext ctx, SplaAmdGemmMod
 SPLA_EXPORT SplaError spla_amd_ctx_set_split_tolerance(SplaContext ctx, double tolerance);
 
 /*
+ * Largest device workspace in bytes (residue planes and INT8 GEMM output) of
+ * one split-precision product; 0 restores the default, a quarter of the device
+ * memory. A product that needs more runs its reduction in chunks of at most
+ * this workspace, each chunk a split product of its own added into C in FP64
+ * (beta on the first chunk only); one that cannot be chunked (fewer than 4096
+ * reduction indices fit) runs on the FP64 kernels (reason 8).
+ */
+SPLA_EXPORT SplaError spla_amd_ctx_set_split_workspace_limit(SplaContext ctx, size_t bytes);
+
+/*
  * Outcome of the context's last double / complex-double product: used = 1 if
  * it ran in split precision; reason = 0 when used, else why FP64 ran
  * (1 non-finite input, 2 zero operand, 3 exponent range too wide, 4 too many
  * moduli, 5 norm or reduction length out of range, 6 FP64 expected faster, 7 split
- * mode off);
+ * mode off, 8 split workspace over the limit and not chunkable, or allocation
+ * failed; for a product run in reduction chunks: the reason of a chunk that ran on
+ * the FP64 kernels);
  * moduli, bitsA and bitsB describe the plan; bound is its estimated normwise
  * error bound. Any output pointer may be NULL.
  */

@@ -104,6 +104,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("spla_amd_ctx_set_stream_chunk_bytes", c_void_p, ctypes.c_size_t)
     fn("spla_amd_ctx_set_gemm_mode", c_void_p, c_int)
     fn("spla_amd_ctx_set_split_tolerance", c_void_p, c_double)
+    fn("spla_amd_ctx_set_split_workspace_limit", c_void_p, ctypes.c_size_t)
     fn("spla_amd_ctx_set_timing", c_void_p, c_int)
     fn("spla_amd_ctx_last_timing", c_void_p, P(c_double))
     fn("spla_timer_start", c_int, ctypes.c_char_p)

@@ -212,6 +212,10 @@ class Context:
     def set_split_tolerance(self, tol: float) -> None:
         check(self._lib.spla_amd_ctx_set_split_tolerance(self._h, float(tol)))
 
+    def set_split_workspace_limit(self, nbytes: int) -> None:
+        """Largest split-precision workspace (bytes; 0: a quarter of the device memory)."""
+        check(self._lib.spla_amd_ctx_set_split_workspace_limit(self._h, int(nbytes)))
+
     def last_product_info(self) -> dict:
         """Form of the last double / complex-double product (split plan, error bound)."""
         v = [ctypes.c_int() for _ in range(5)]

@@ -37,6 +37,12 @@ SPLA_EXPORT SplaError spla_amd_ctx_set_split_tolerance(SplaContext ctx, double t
                    .set_split_tolerance(tolerance))
 }
 
+SPLA_EXPORT SplaError spla_amd_ctx_set_split_workspace_limit(SplaContext ctx, size_t bytes) {
+  if (!ctx) return SPLA_INVALID_HANDLE_ERROR;
+  SPLA_C_GUARD(spla::InternalAccess::ctx(*reinterpret_cast<spla::Context*>(ctx))
+                   .set_split_workspace_limit(bytes))
+}
+
 SPLA_EXPORT SplaError spla_amd_ctx_last_product_info(SplaContext ctx, int* used, int* reason,
                                                      int* moduli, int* bitsA, int* bitsB,
                                                      double* bound) {

@@ -83,7 +83,14 @@ void ContextInternal::ensure_device_resources() {
     hipDeviceProp_t prop;
     spla_amd::hip_check(hipGetDeviceProperties(&prop, deviceId_));
     numCUs_ = prop.multiProcessorCount;
+    deviceMemory_ = prop.totalGlobalMem;
   }
+}
+
+std::size_t ContextInternal::split_workspace_limit() {
+  if (splitWorkspaceLimit_) return splitWorkspaceLimit_;
+  ensure_device_resources();
+  return deviceMemory_ / 4;
 }
 
 hipStream_t ContextInternal::compute_stream() {

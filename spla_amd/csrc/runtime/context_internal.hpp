@@ -76,6 +76,10 @@ public:
   void set_split_tolerance(double t) { splitTolerance_ = t; }
   int split_max_moduli() const { return splitMaxModuli_; }
   void set_split_max_moduli(int v) { splitMaxModuli_ = v; }
+  // Largest device workspace (residue planes and INT8 GEMM output) of one split
+  // product: the caller's value, or a quarter of the device memory (0 = default).
+  std::size_t split_workspace_limit();
+  void set_split_workspace_limit(std::size_t b) { splitWorkspaceLimit_ = b; }
   spla_amd::SplitReport& last_split() { return lastSplit_; }
   spla_amd::PhaseTimer& timer() { return timer_; }
 
@@ -100,11 +104,13 @@ private:
   hipStream_t copy_ = nullptr;
   hipStream_t batch_[kBatchStreams] = {nullptr, nullptr, nullptr};
   int numCUs_ = 0;
+  std::size_t deviceMemory_ = 0;
 
   spla_amd::Tuner tuner_;
   int gemmMode_ = 0;
   double splitTolerance_ = 1e-14;
   int splitMaxModuli_ = 20;
+  std::size_t splitWorkspaceLimit_ = 0;
   spla_amd::SplitReport lastSplit_;
   spla_amd::PhaseTimer timer_;
 };

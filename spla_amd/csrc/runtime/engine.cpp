@@ -174,7 +174,7 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
   if constexpr (!f32) {
     if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
         enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
-                                 fill, plan.estSeconds, stream, keep))
+                                 fill, plan.estSeconds, stream, keep, reuse))
       return;
   }
   d.splits = plan.splits;
@@ -296,7 +296,9 @@ void gemm_gpu_f64(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, in
     keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(m) * n * sizeof(T));
     dC = keep.back().get<T>();
     dldc = m;
-    if (!betaZero)
+    // With a fill mode, tiles outside the triangle are not written but the whole
+    // of dC is copied back: it must hold the caller's C there even when beta == 0.
+    if (!betaZero || fill.mode != SPLA_FILL_MODE_FULL)
       hip_check(hipMemcpy2DAsync(dC, m * sizeof(T), C, ldc * sizeof(T), m * sizeof(T), n,
                                  hipMemcpyHostToDevice, stream));
   }
@@ -419,7 +421,7 @@ void gemm_gpu_f32(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, in
     keep.emplace_back(ctx.gpu_memory(), static_cast<std::size_t>(m) * n * sizeof(TF));
     dC = keep.back().get<TF>();
     ldC = m;
-    if (!(beta == TF(0)))
+    if (!(beta == TF(0)) || fill.mode != SPLA_FILL_MODE_FULL)  // see gemm_gpu_f64
       hip_check(hipMemcpy2DAsync(dC, m * sizeof(TF), C, ldc * sizeof(TF), m * sizeof(TF), n,
                                  hipMemcpyHostToDevice, stream));
   }
@@ -470,10 +472,9 @@ void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t 
                        T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C,
                        int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
                        hipStream_t stream, std::vector<Block>& keep, Block* reuse,
-                       bool trySplit) {
-  // the block products of a batch share the device with the other streams
+                       bool trySplit, bool wholeChip) {
   enqueue_device_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill,
-                            stream, keep, reuse, trySplit, /*wholeChip=*/false);
+                            stream, keep, reuse, trySplit, wholeChip);
 }
 
 double fp64_estimate_seconds(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
@@ -487,12 +488,12 @@ template void enqueue_on_device<double>(SplaOperation, SplaOperation, int64_t, i
                                         double, const double*, int64_t, const double*, int64_t,
                                         double, double*, int64_t, spla::ContextInternal&,
                                         const FillSpec&, hipStream_t, std::vector<Block>&, Block*,
-                                        bool);
+                                        bool, bool);
 template void enqueue_on_device<std::complex<double>>(
     SplaOperation, SplaOperation, int64_t, int64_t, int64_t, std::complex<double>,
     const std::complex<double>*, int64_t, const std::complex<double>*, int64_t,
     std::complex<double>, std::complex<double>*, int64_t, spla::ContextInternal&, const FillSpec&,
-    hipStream_t, std::vector<Block>&, Block*, bool);
+    hipStream_t, std::vector<Block>&, Block*, bool, bool);
 
 template void gemm_gpu<float>(SplaOperation, SplaOperation, int64_t, int64_t, int64_t, float,
                               const float*, int64_t, const float*, int64_t, float, float*, int64_t,

@@ -44,13 +44,15 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
 // synchronization (T: double or std::complex<double>). Workspace blocks are
 // appended to `keep` and must live until the stream has drained; `reuse`
 // (optional) is one split-K workspace shared by products on the same stream.
-// trySplit = false runs the FP64 kernels whatever the context's mode.
+// trySplit = false runs the FP64 kernels whatever the context's mode;
+// wholeChip = false plans for a device shared with other streams (the block
+// products of a batch).
 template <typename T>
 void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                        T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C,
                        int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
                        hipStream_t stream, std::vector<Block>& keep, Block* reuse,
-                       bool trySplit = true);
+                       bool trySplit = true, bool wholeChip = false);
 
 // Estimated seconds of the FP64 kernels for one product (the tuner's model),
 // which the automatic mode compares with the split form; wholeChip = false for

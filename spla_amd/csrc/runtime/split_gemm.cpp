@@ -4,6 +4,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <type_traits>
 
 #include "kernels/launch.hpp"
@@ -11,6 +12,7 @@
 #include "runtime/hip_check.hpp"
 #include "runtime/split_plan.hpp"
 #include "runtime/timer.hpp"
+#include "spla/exceptions.hpp"
 
 namespace spla_amd {
 namespace {
@@ -322,10 +324,49 @@ void SplitProduct<T>::enqueue_rest(hipStream_t stream, void* workspace, std::vec
 }
 
 template <typename T>
+int64_t SplitProduct<T>::max_chunk_k(std::size_t limit) const {
+  // the planes grow with the reduction length, the GEMM output does not
+  const std::size_t planes = round_up(static_cast<int64_t>(G_) * mpad_ * kpad_, 256) +
+                             round_up(static_cast<int64_t>(G_) * npad_ * kpad_, 256);
+  const std::size_t out = workspace_bytes() - planes;
+  if (limit <= out + 512) return 0;
+  const int64_t perK = static_cast<int64_t>(G_) * (mpad_ + npad_);
+  return (static_cast<int64_t>(limit - out - 512) / perK) / kKpadAlign * kKpadAlign;
+}
+
+namespace {
+
+// Device workspace of at least `bytes`: the caller's reusable block (grown when
+// too small; the stream has drained, so the old one may go back to the pool) or
+// a block appended to `keep`. nullptr when the device memory is exhausted.
+void* split_workspace(spla::ContextInternal& ctx, std::size_t bytes, Block* reuse,
+                      std::vector<Block>& keep) {
+  try {
+    if (reuse) {
+      if (reuse->bytes() < bytes) {
+        reuse->reset();
+        *reuse = Block(ctx.gpu_memory(), bytes);
+      }
+      return reuse->get();
+    }
+    keep.emplace_back(ctx.gpu_memory(), bytes);
+    return keep.back().get();
+  } catch (const spla::GPUAllocationError&) {
+    return nullptr;
+  }
+}
+
+// Shortest reduction chunk worth a split product of its own.
+constexpr int64_t kMinChunkK = 4096;
+
+}  // namespace
+
+template <typename T>
 bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                            T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                            T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
-                           double fp64Seconds, hipStream_t stream, std::vector<Block>& keep) {
+                           double fp64Seconds, hipStream_t stream, std::vector<Block>& keep,
+                           Block* reuse) {
   SplitProduct<T> p(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
   bool ok = (ctx.gemm_mode() != GemmModeAuto || p.may_beat_fp64(fp64Seconds)) &&
             p.enqueue_stats(stream);
@@ -334,9 +375,75 @@ bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int6
     hip_check(hipStreamSynchronize(stream));
     ok = p.plan(fp64Seconds);
   }
-  if (ok) p.enqueue_rest(stream, nullptr, keep);
-  ctx.last_split() = p.report();
-  return ok;
+  if (!ok) {
+    ctx.last_split() = p.report();
+    return false;
+  }
+  const std::size_t limit = ctx.split_workspace_limit();
+  if (p.workspace_bytes() <= limit) {
+    void* ws = split_workspace(ctx, p.workspace_bytes(), reuse, keep);
+    if (!ws) {
+      SplitReport r = p.report();
+      r.used = 0;
+      r.reason = SplitNoWorkspace;
+      ctx.last_split() = r;
+      return false;
+    }
+    p.enqueue_rest(stream, ws, keep);
+    ctx.last_split() = p.report();
+    return true;
+  }
+
+  // Over the workspace limit: the reduction runs in chunks, each a split product
+  // of its own (statistics, plan, residues, INT8 GEMM, reconstruction) that adds
+  // into C in FP64, beta on the first chunk only. Each chunk meets the tolerance
+  // relative to its own ||A_c|| ||B_c||, and sum_c ||A_c|| ||B_c|| <= ||A|| ||B||.
+  const int64_t kmax = p.max_chunk_k(limit);
+  if (kmax < kMinChunkK) {
+    SplitReport r = p.report();
+    r.used = 0;
+    r.reason = SplitNoWorkspace;
+    ctx.last_split() = r;
+    return false;
+  }
+  const int64_t chunks = (k + kmax - 1) / kmax;
+  const int64_t kc = std::min(kmax, round_up((k + chunks - 1) / chunks, kKpadAlign));
+  Block own;
+  Block* ws = reuse ? reuse : &own;
+  SplitReport agg;
+  agg.used = 1;
+  agg.reason = SplitOk;
+  for (int64_t k0 = 0; k0 < k; k0 += kc) {
+    const int64_t kcur = std::min(kc, k - k0);
+    const T* cA = opA == SPLA_OP_NONE ? A + k0 * lda : A + k0;
+    const T* cB = opB == SPLA_OP_NONE ? B + k0 : B + k0 * ldb;
+    const T cBeta = k0 == 0 ? beta : T(1);
+    SplitProduct<T> q(opA, opB, m, n, kcur, alpha, cA, lda, cB, ldb, cBeta, C, ldc, ctx, fill);
+    bool split = q.enqueue_stats(stream);
+    if (split) {
+      timer::LibraryScope scope("split_plan");
+      hip_check(hipStreamSynchronize(stream));  // also frees the previous chunk's workspace
+      // the whole product was found worth splitting; chunks are not re-judged
+      split = q.plan(std::numeric_limits<double>::infinity()) && q.workspace_bytes() <= limit;
+    }
+    void* w = split ? split_workspace(ctx, q.workspace_bytes(), ws, keep) : nullptr;
+    if (w) {
+      q.enqueue_rest(stream, w, keep);
+      agg.moduli = std::max(agg.moduli, q.report().moduli);
+      agg.bitsA = std::max(agg.bitsA, q.report().bitsA);
+      agg.bitsB = std::max(agg.bitsB, q.report().bitsB);
+      agg.bound = std::max(agg.bound, q.report().bound);
+    } else {
+      // this chunk on the FP64 kernels (same beta rule)
+      enqueue_on_device<T>(opA, opB, m, n, kcur, alpha, cA, lda, cB, ldb, cBeta, C, ldc, ctx,
+                           fill, stream, keep, nullptr, /*trySplit=*/false, /*wholeChip=*/true);
+      agg.used = 0;
+      agg.reason = q.report().reason != SplitOk ? q.report().reason : SplitNoWorkspace;
+    }
+  }
+  if (!reuse) keep.push_back(std::move(own));
+  ctx.last_split() = agg;
+  return true;
 }
 
 template class SplitProduct<double>;
@@ -346,11 +453,11 @@ template bool enqueue_split_product<double>(SplaOperation, SplaOperation, int64_
                                             int64_t, double, const double*, int64_t,
                                             const double*, int64_t, double, double*, int64_t,
                                             spla::ContextInternal&, const FillSpec&, double,
-                                            hipStream_t, std::vector<Block>&);
+                                            hipStream_t, std::vector<Block>&, Block*);
 template bool enqueue_split_product<std::complex<double>>(
     SplaOperation, SplaOperation, int64_t, int64_t, int64_t, std::complex<double>,
     const std::complex<double>*, int64_t, const std::complex<double>*, int64_t,
     std::complex<double>, std::complex<double>*, int64_t, spla::ContextInternal&, const FillSpec&,
-    double, hipStream_t, std::vector<Block>&);
+    double, hipStream_t, std::vector<Block>&, Block*);
 
 }  // namespace spla_amd

@@ -44,6 +44,9 @@ public:
   bool plan(double fp64Seconds);
   // Device bytes of the residue planes and INT8 GEMM output (after plan()).
   std::size_t workspace_bytes() const;
+  // Longest reduction chunk (a multiple of 256) whose workspace, at this plan's
+  // moduli, fits `limit` bytes; 0 when none does (after plan()).
+  int64_t max_chunk_k(std::size_t limit) const;
   // `workspace`: at least workspace_bytes(), 256-byte aligned, or nullptr to
   // allocate it into `keep`; the other buffers are appended to `keep`, which
   // must live until the stream has drained.
@@ -72,13 +75,24 @@ private:
 };
 
 // The single-product form: the three stages with one synchronization of
-// `stream` in between. Returns true when the split form was enqueued, false
-// (nothing but the statistics enqueued) when the product falls back to the
+// `stream` in between. Returns true when the product was enqueued (in split
+// precision, or in reduction chunks of which some may run on the FP64 kernels),
+// false (nothing but the statistics enqueued) when the product falls back to the
 // FP64 kernels. The report is stored in the context.
+//
+// The workspace (residue planes and GEMM output) is kept within the context's
+// split workspace limit: a product whose workspace exceeds it runs in reduction
+// chunks of at most the limit, one after the other in one workspace; a product
+// that cannot be chunked (the limit holds less than 4096 reduction indices) or
+// whose workspace cannot be allocated falls back (reason SplitNoWorkspace).
+// `reuse` (optional): a workspace block shared by the products of one stream,
+// grown when too small, so that the streamed k-chunks of host-resident operands
+// hold one workspace between them.
 template <typename T>
 bool enqueue_split_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                            T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                            T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
-                           double fp64Seconds, hipStream_t stream, std::vector<Block>& keep);
+                           double fp64Seconds, hipStream_t stream, std::vector<Block>& keep,
+                           Block* reuse = nullptr);
 
 }  // namespace spla_amd

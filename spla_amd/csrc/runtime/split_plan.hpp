@@ -47,7 +47,8 @@ enum SplitReason : int {
   SplitTooManyModuli = 4,
   SplitRange = 5,         // norms outside the double range
   SplitNotFaster = 6,     // automatic mode: the FP64 kernel is expected to be faster
-  SplitDisabled = 7
+  SplitDisabled = 7,
+  SplitNoWorkspace = 8    // workspace over the limit and not chunkable, or allocation failed
 };
 
 struct SplitPlan {
