@@ -14,8 +14,18 @@ bound of 1e-13 (--split-tolerance), the accuracy the metric requires; the
 library's default tolerance is 1e-14 (one more modulus on this workload).
 
 value: FP64-accurate TFLOP/s (8*m*n*k / seconds per step) of the fastest form
-whose measured normwise error ||C - C_dd|| / (||A|| ||B||) (sampled entries
-against the double-double oracle) is <= 1e-13. Every form run is reported.
+whose measured normwise error ||C - C_ref|| / (||A|| ||B||) is <= 1e-13, from a
+64 x 64 grid of sampled entries (4096) against error-free products and
+compensated sums computed on the device. Every form run is reported.
+
+workloads: after the headline, the default run measures every other
+configuration in the same process with fewer steps, one entry each: the headline
+with A, B, C in pinned host memory (PCIe-inclusive; plain, split, automatic),
+pzgemm_sbs with mLocal = 100000 (plain, split), the real A^T B with an exponent
+spread (plain, split, and a forced FP64 fall-back), 512 uneven ring-step blocks
+through the batch API (plain, split, automatic), the headline in complex single
+precision and two HBM-bound small m, n products. --no-workloads skips them;
+--workload NAME runs one of them alone.
 
 roofline: the dominant kernel of that form (FP64 GEMM kernel, or the batched
 INT8 GEMM of the split form), its duration from HIP events the library records
@@ -24,7 +34,8 @@ published peak (FP64 matrix 78.6 TFLOP/s; INT8 dense 5.0 POP/s).
 
 cpu_baseline: SPLA's CPU path (one host-BLAS zgemm, as in
 src/gemm/gemm_host.cpp:55-72) on a k-truncated sample of the same workload,
-timed on this host's cores.
+timed on this host's cores; "config1_full" times the CPU-runnable configuration
+(m = n = 256, k = 4096) in full.
 """
 
 from __future__ import annotations
@@ -52,6 +63,7 @@ INT8_PEAK_TOPS = 5000.0
 # copy (reads x 2.0, writes x 1.0). They change only when the kernels or the split
 # plan (number of moduli) change.
 HBM_TRAFFIC_BYTES = {"split": 51.66e9 + 1.41e9, "plain": 116.67e9 + 0.07e9}
+PROFILED_MODULI = 14  # moduli of the split plan the counters were taken on
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
 
@@ -97,8 +109,8 @@ def uniform_device(seed: int, mid: int, count: int, complex_: bool, device, chun
 # ---------------------------------------------------------------------------
 
 def _two_prod(x, y):
-    """Error-free product x * y = p + e (Veltkamp splitting, no FMA needed)."""
-    import numpy as np
+    """Error-free product x * y = p + e (Dekker, Veltkamp splitting, no FMA needed).
+    x, y: numpy arrays or torch tensors (CPU or GPU), float64."""
 
     def split(v):
         c = v * 134217729.0  # 2^27 + 1
@@ -114,57 +126,101 @@ def _two_prod(x, y):
 
 def _accurate_sum(v):
     """Sum of v (last axis) with error-free pairwise TwoSum steps: the rounding
-    errors of every addition are kept and added at the end."""
-    import numpy as np
-
-    v = np.asarray(v, dtype=np.float64)
-    errs = np.zeros(v.shape[:-1])
+    errors of every addition are kept and added at the end. numpy or torch."""
+    try:
+        import torch
+        is_torch = isinstance(v, torch.Tensor)
+    except ImportError:  # pragma: no cover
+        is_torch = False
+    if not is_torch:
+        import numpy as np
+        v = np.asarray(v, dtype=np.float64)
+    errs = None
     while v.shape[-1] > 1:
         if v.shape[-1] % 2:
-            v = np.concatenate([v, np.zeros(v.shape[:-1] + (1,))], axis=-1)
+            if is_torch:
+                import torch
+                v = torch.cat([v, torch.zeros(v.shape[:-1] + (1,), dtype=v.dtype,
+                                              device=v.device)], dim=-1)
+            else:
+                v = np.concatenate([v, np.zeros(v.shape[:-1] + (1,))], axis=-1)
         a, b = v[..., 0::2], v[..., 1::2]
         s = a + b
         bb = s - a
-        errs = errs + ((a - (s - bb)) + (b - bb)).sum(axis=-1)
+        e = ((a - (s - bb)) + (b - bb)).sum(-1)
+        errs = e if errs is None else errs + e
         v = s
-    return v[..., 0] + errs
+    return v[..., 0] + (errs if errs is not None else 0.0)
 
 
 def _accurate_dot(x, y):
-    """sum_l x[..., l] * y[..., l] to about double-double accuracy."""
-    import numpy as np
-
+    """sum_l x[..., l] * y[..., l] to about double-double accuracy (numpy or torch)."""
     p, e = _two_prod(x, y)
+    try:
+        import torch
+        if isinstance(p, torch.Tensor):
+            return _accurate_sum(torch.cat([p, e], dim=-1))
+    except ImportError:  # pragma: no cover
+        pass
+    import numpy as np
     return _accurate_sum(np.concatenate([p, e], axis=-1))
 
 
-def sampled_error(dA, dB, dC0, dC, k, m, n, samples=48, seed=3):
-    """Normwise error estimate ||C - C_ref|| / (||A|| ||B||) from sampled entries;
-    C_ref = alpha A^H B + beta C0 with error-free products and compensated sums."""
-    import numpy as np
+def accurate_products(X, Y, rows_per_pass=None):
+    """Every entry sum_l X[r, l] * Y[c, l] of X (R x K) and Y (C x K), float64 or
+    complex128 torch tensors on one device, accurate to about double-double.
+    Complex: Re = [Xr, -Xi] . [Yr, Yi], Im = [Xr, Xi] . [Yi, Yr] (real embedding)."""
     import torch
 
+    if X.is_complex():
+        Xr, Xi, Yr, Yi = X.real, X.imag, Y.real, Y.imag
+        re = accurate_products(torch.cat([Xr, -Xi], 1), torch.cat([Yr, Yi], 1), rows_per_pass)
+        im = accurate_products(torch.cat([Xr, Xi], 1), torch.cat([Yi, Yr], 1), rows_per_pass)
+        return torch.complex(re, im)
+    R, C, K = X.shape[0], Y.shape[0], X.shape[1]
+    rb = rows_per_pass or max(1, min(R, (1 << 27) // max(1, C * K)))
+    out = torch.empty(R, C, dtype=torch.float64, device=X.device)
+    for r0 in range(0, R, rb):
+        x = X[r0:r0 + rb, None, :]
+        out[r0:r0 + rb] = _accurate_dot(x.expand(-1, C, -1), Y[None, :, :].expand(x.shape[0], -1, -1))
+    return out
+
+
+def sampled_rows_cols(m, n, count, seed):
+    """count x count grid of sampled entries: sorted distinct rows and columns."""
+    import numpy as np
+
     rng = np.random.default_rng(seed)
-    rows = rng.integers(0, m, samples)
-    cols = rng.integers(0, n, samples)
-    ur, rinv = np.unique(rows, return_inverse=True)
-    uc, cinv = np.unique(cols, return_inverse=True)
-    A_cols = dA.view(m, k)[torch.from_numpy(ur).to(dA.device)].cpu().numpy()  # (len(ur), k)
-    B_cols = dB.view(n, k)[torch.from_numpy(uc).to(dB.device)].cpu().numpy()
-    a = A_cols[rinv]
-    b = B_cols[cinv]
-    ar, ai, br, bi = a.real, a.imag, b.real, b.imag
-    # conj(a) . b = (ar br + ai bi) + i (ar bi - ai br)
-    re = _accurate_dot(np.concatenate([ar, ai], axis=-1), np.concatenate([br, bi], axis=-1))
-    im = _accurate_dot(np.concatenate([ar, -ai], axis=-1), np.concatenate([bi, br], axis=-1))
-    C0 = dC0.view(n, m).cpu().numpy()  # column-major: C0[j, i]
-    got = dC.view(n, m).cpu().numpy()
-    ref = ALPHA * (re + 1j * im) + BETA * C0[cols, rows]
-    diff = got[cols, rows] - ref
-    est = np.sqrt(np.mean(np.abs(diff) ** 2) * m * n)
-    nA = torch.linalg.vector_norm(dA).item()
-    nB = torch.linalg.vector_norm(dB).item()
-    return float(est / (nA * nB)), samples
+    rows = np.sort(rng.choice(m, size=min(m, count), replace=False))
+    cols = np.sort(rng.choice(n, size=min(n, count), replace=False))
+    return rows, cols
+
+
+def normwise_from_samples(got, ref, m, n, nAB):
+    """||C - C_ref||_F / (||A|| ||B||) estimated from sampled entries (RMS x sqrt(m n))."""
+    import torch
+
+    d = (got - ref).abs().double()
+    return float(torch.sqrt((d * d).mean() * (m * n)).item() / nAB)
+
+
+def sampled_error(dA, dB, dC0, dC, k, m, n, samples=64, seed=3):
+    """Normwise error estimate ||C - C_ref|| / (||A|| ||B||) of the headline product
+    from a samples x samples grid of entries (4096 by default); C_ref = alpha A^H B +
+    beta C0 with error-free products and compensated sums, on the device."""
+    import torch
+
+    rows, cols = sampled_rows_cols(m, n, samples, seed)
+    ri = torch.from_numpy(rows).to(dA.device)
+    ci = torch.from_numpy(cols).to(dA.device)
+    X = dA.view(m, k)[ri].conj()  # rows of A^H
+    Y = dB.view(n, k)[ci]
+    ab = accurate_products(X, Y)  # (rows, cols)
+    C0 = dC0.view(n, m)[ci][:, ri].T  # C0[i, j]
+    ref = ALPHA * ab + BETA * C0
+    got = dC.view(n, m)[ci][:, ri].T
+    nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
+    return normwise_from_samples(got, ref, m, n, nAB), len(rows) * len(cols)
 
 
 def cpu_baseline(m, n, k_full, target_seconds=12.0):
@@ -202,6 +258,7 @@ def cpu_baseline(m, n, k_full, target_seconds=12.0):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         cores = os.cpu_count()
+    cfg1 = cpu_config1(threads)
     return {
         "value": round(rate, 4),
         "unit": "TFLOP/s",
@@ -211,11 +268,57 @@ def cpu_baseline(m, n, k_full, target_seconds=12.0):
                    f"{t2:.2f} s; extrapolated full-k step {t2 * k_full / k2:.1f} s; "
                    f"all threads at k={k1}: {8.0 * m * n * k1 / t1 / 1e12:.4f} TFLOP/s{one}; "
                    f"host cores visible {cores}"),
+        "config1_full": cfg1,
     }
+
+
+def cpu_config1(threads):
+    """The CPU-runnable configuration in full: pzgemm_ssb on one rank, m = n = 256,
+    k = 4096, A^H B, alpha = 1, beta = 0 (at one rank SPLA's CPU path is one host
+    zgemm, src/gemm/gemm_host.cpp:55-72); 1 warm-up, median of 7, all threads and 1."""
+    import numpy as np
+
+    import oracle
+
+    m = n = 256
+    k = 4096
+    A = oracle.uniform(SEED, 1, k * m, complex_=True)
+    B = oracle.uniform(SEED, 2, k * n, complex_=True)
+
+    def med(nthreads):
+        if nthreads:
+            oracle.lib().cpu_path_set_threads(nthreads)
+        C = np.zeros(m * n, dtype=np.complex128)
+        oracle.cpu_path_gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.0, C, m)
+        ts = []
+        for _ in range(7):
+            t0 = time.perf_counter()
+            oracle.cpu_path_gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.0, C, m)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    try:
+        t_all = med(0)
+        t_one = med(1)
+    finally:
+        if threads:
+            oracle.lib().cpu_path_set_threads(threads)
+    flops = 8.0 * m * n * k
+    return {"workload": "spla_pzgemm_ssb m=n=256 k=4096 A^H B complex, alpha=1 beta=0, "
+                        "64x64 blocks on 1x1, SPLA's CPU path (host zgemm)",
+            "ms": round(t_all * 1e3, 3), "tflops": round(flops / t_all / 1e12, 4),
+            "threads": threads, "ms_1_thread": round(t_one * 1e3, 3),
+            "tflops_1_thread": round(flops / t_one / 1e12, 4)}
 
 
 WORKLOADS = ("ssb2048", "sbs100k", "ssb_real_spread", "batch512", "ssb2048_host",
              "ssb2048_c64", "ssb_small_mn")
+# side workloads of the default run, in order (each a key of the JSON line's "workloads")
+DEFAULT_SIDE = ("ssb2048_host", "sbs100k", "ssb_real_spread", "batch512", "ssb2048_c64",
+                "ssb_small_mn")
+SIDE_FORMS = {"ssb2048_host": ["plain", "split", "auto"], "sbs100k": ["plain", "split"],
+              "ssb_real_spread": ["plain", "split"], "batch512": ["plain", "split", "auto"],
+              "ssb2048_c64": ["plain"], "ssb_small_mn": ["plain"]}
 
 
 def _timed(fn, steps, warmup, stream):
@@ -249,241 +352,385 @@ def _phases(ctx, fn) -> dict:
     return {k: round(v, 3) for k, v in ph.items() if v > 0}
 
 
-def side_workload(args) -> int:
-    """The other parity workloads: sbs with a tall local A, a real ssb product with a
-    spread of exponents (and a range that forces the FP64 fall-back), 512 uneven
-    ring-step blocks through the batch API, and the headline product with
-    host-resident operands (PCIe-inclusive). One JSON line each."""
+def _modes(spla):
+    return {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8,
+            "auto": spla.GemmMode.AUTO}
+
+
+def _info_keys(info) -> dict:
+    return {"split_used": info["split"], "fallback_reason": info["reason"],
+            "moduli": info["moduli"], "bits": [info["bits_a"], info["bits_b"]],
+            "error_bound": info["bound"]}
+
+
+def _release(*tensors) -> None:
+    import torch
+
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def _ssb_reference(dA, dB, k, m, n, rows, cols, conj=True):
+    """Accurate A^H B (conj) or A^T B entries at rows x cols (A k x m, B k x n)."""
+    import torch
+
+    ri = torch.from_numpy(rows).to(dA.device)
+    ci = torch.from_numpy(cols).to(dA.device)
+    X = dA.view(m, k)[ri]
+    if conj and X.is_complex():
+        X = X.conj()
+    return accurate_products(X.resolve_conj(), dB.view(n, k)[ci])
+
+
+def _gather(dC, m, n, rows, cols):
+    """Entries C[rows, cols] of a column-major m x n device matrix, as (rows, cols)."""
+    import torch
+
+    ri = torch.from_numpy(rows).to(dC.device)
+    ci = torch.from_numpy(cols).to(dC.device)
+    return dC.view(n, m)[ci][:, ri].T
+
+
+def workload_ssb2048_host(spla, dev, stream, steps, warmup, forms, tol):
+    """the headline product with A, B and C in pinned host memory
+    (k-chunked streaming over PCIe overlapped with the kernels). PCIe-inclusive."""
+    import torch
+
+    m, n, k = M, N, K
+    hA = torch.empty(k * m, dtype=torch.complex128, pin_memory=True)
+    hB = torch.empty(k * n, dtype=torch.complex128, pin_memory=True)
+    hC = torch.empty(m * n, dtype=torch.complex128, pin_memory=True)
+    dA = uniform_device(SEED, 1, k * m, True, dev)
+    hA.copy_(dA)
+    nA = torch.linalg.vector_norm(dA).item()
+    rows, cols = sampled_rows_cols(m, n, 32, 7)
+    ab = None
+    dB = uniform_device(SEED, 2, k * n, True, dev)
+    hB.copy_(dB)
+    nB = torch.linalg.vector_norm(dB).item()
+    ab = _ssb_reference(dA, dB, k, m, n, rows, cols)
+    del dA, dB
+    dC0 = uniform_device(SEED, 3, m * n, True, dev)
+    hC0 = dC0.cpu()
+    ref = ALPHA * ab + BETA * _gather(dC0, m, n, rows, cols)
+    del dC0
+    _release()
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    ctx.set_split_tolerance(tol)
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 256,
+                                                             256)
+    flops = 8.0 * m * n * k
+    out = {"config": {"workload": "spla_pzgemm_ssb A^H B complex m=n=2048 k=200000, beta=0.5, "
+                                  "A, B, C in pinned host memory (PCIe-inclusive)",
+                      "split_tolerance": tol},
+           "dtype": "c128", "forms": {}}
+    for form in forms:
+        ctx.set_gemm_mode(_modes(spla)[form])
+        run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, hA, k, hB, k,
+                                     BETA, hC, m, 0, 0, dist, ctx)
+        hC.copy_(hC0)
+        run()
+        got = _gather(hC.to(dev), m, n, rows, cols)
+        err = normwise_from_samples(got, ref, m, n, nA * nB)
+        info = ctx.last_product_info()
+        wall, _ = _timed(run, steps, warmup, stream)
+        out["forms"][form] = {"tflops_pcie_inclusive": round(flops / wall / 1e9, 3),
+                              "ms_per_step": round(wall, 3), "normwise_error": err,
+                              "error_samples": len(rows) * len(cols), **_info_keys(info)}
+        log(f"ssb2048_host {form}: {out['forms'][form]}")
+    ctx.close()
+    del hA, hB, hC
+    _release()
+    return out
+
+
+def workload_sbs100k(spla, dev, stream, steps, warmup, forms, tol):
+    """pzgemm_sbs, mLocal = 100000, n = k = 4096, device-resident."""
+    import torch
+
+    mL, n, k = 100000, 4096, 4096
+    dA = uniform_device(SEED, 1, mL * k, True, dev)
+    dB = uniform_device(SEED, 2, k * n, True, dev)
+    dC = torch.zeros(mL * n, dtype=torch.complex128, device=dev)
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    ctx.set_split_tolerance(tol)
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 128,
+                                                             128)
+    flops = 8.0 * mL * n * k
+    rows, cols = sampled_rows_cols(mL, n, 32, 3)
+    X = dA.view(k, mL)[:, torch.from_numpy(rows).to(dev)].T.contiguous()  # rows of A
+    Y = dB.view(n, k)[torch.from_numpy(cols).to(dev)]                     # columns of B
+    ref = accurate_products(X, Y)
+    nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
+    out = {"config": {"workload": "spla_pzgemm_sbs complex, mLocal=100000 n=4096 k=4096, "
+                                  "device-resident, beta=0", "dist": "block cyclic 128x128 on 1x1",
+                      "split_tolerance": tol},
+           "dtype": "c128", "forms": {}}
+    for form in forms:
+        ctx.set_gemm_mode(_modes(spla)[form])
+        run = lambda: spla.pgemm_sbs(mL, n, k, 1.0, dA, mL, dB, k, 0, 0, dist, 0.0, dC, mL, ctx)
+        wall, ev = _timed(run, steps, warmup, stream)
+        err = normwise_from_samples(_gather(dC, mL, n, rows, cols), ref, mL, n, nAB)
+        info = ctx.last_product_info()
+        out["forms"][form] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
+                              "device_ms_per_step": round(ev, 3), "normwise_error": err,
+                              "error_samples": len(rows) * len(cols), **_info_keys(info),
+                              "phase_ms": _phases(ctx, run)}
+        log(f"sbs100k {form}: {out['forms'][form]}")
+    ctx.close()
+    del dA, dB, dC
+    _release()
+    return out
+
+
+def workload_ssb_real_spread(spla, dev, stream, steps, warmup, forms, tol):
+    """pdgemm_ssb A^T B real, m = n = 2048, k = 2e5, entries
+    uniform[-1,1] * 2^e with e uniform in [-10, 10]; a wider range; and rows with one
+    dominant entry, which the split form's bound cannot serve (FP64 fall-back)."""
+    import torch
+
+    m = n = 2048
+    k = 200000
+
+    def spread(mid, count, lo, hi):
+        x = uniform_device(SEED, mid, count, False, dev)
+        e = (uniform_device(SEED, mid + 100, count, False, dev) + 1.0) * 0.5 * (hi - lo + 1)
+        e = torch.clamp(torch.floor(e) + lo, lo, hi).to(torch.int32)
+        return torch.ldexp(x, e)
+
+    def dominant(mid, rows_):
+        # one entry of magnitude ~1 per row (column of A), the rest ~1e-40
+        x = uniform_device(SEED, mid, k * rows_, False, dev) * 1e-40
+        x.view(rows_, k)[:, 0] = 1.0
+        return x
+
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    ctx.set_split_tolerance(tol)
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 256,
+                                                             256)
+    flops = 2.0 * m * n * k
+    rows, cols = sampled_rows_cols(m, n, 32, 3)
+    out = {"config": {"workload": "spla_pdgemm_ssb A^T B real, m=n=2048 k=200000, entries "
+                                  "uniform[-1,1] * 2^e", "dist": "block cyclic 256x256 on 1x1",
+                      "split_tolerance": tol},
+           "dtype": "f64", "forms": {}}
+    cases = (("e in [-10, 10]", lambda mid, r: spread(mid, k * r, -10, 10)),
+             ("e in [-300, 300]", lambda mid, r: spread(mid, k * r, -300, 300)),
+             ("one dominant entry per row of A (forced fall-back)",
+              lambda mid, r: dominant(mid, r) if mid == 11 else spread(mid, k * r, -10, 10)))
+    for case, gen in cases:
+        dA = gen(11, m)
+        dB = gen(12, n)
+        dC = torch.zeros(m * n, dtype=torch.float64, device=dev)
+        ref = _ssb_reference(dA, dB, k, m, n, rows, cols, conj=False)
+        nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
+        case_forms = forms if not case.startswith("one") else [f for f in forms if f != "plain"]
+        for form in case_forms:
+            ctx.set_gemm_mode(_modes(spla)[form])
+            run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.TRANSPOSE, 1.0, dA, k, dB, k, 0.0,
+                                         dC, m, 0, 0, dist, ctx)
+            wall, ev = _timed(run, steps, warmup, stream)
+            err = normwise_from_samples(_gather(dC, m, n, rows, cols), ref, m, n, nAB)
+            info = ctx.last_product_info()
+            key = f"{form}, {case}"
+            out["forms"][key] = {"tflops": round(flops / wall / 1e9, 3),
+                                 "ms_per_step": round(wall, 3), "device_ms_per_step": round(ev, 3),
+                                 "normwise_error": err, "error_samples": len(rows) * len(cols),
+                                 **_info_keys(info), "phase_ms": _phases(ctx, run)}
+            log(f"ssb_real_spread {key}: {out['forms'][key]}")
+        del dA, dB, dC
+        _release()
+    sp = out["forms"].get("split, e in [-10, 10]")
+    pl = out["forms"].get("plain, e in [-10, 10]")
+    if sp and pl:
+        out["split_over_plain"] = round(sp["tflops"] / pl["tflops"], 3)
+    ctx.close()
+    return out
+
+
+def workload_batch512(spla, dev, stream, steps, warmup, forms, tol):
+    """512 uneven complex ring-step blocks (m, n log-uniform in
+    [32, 512], k in [4096, 65536], seed 5; odd blocks LOWER fill) through the batch
+    API, alpha = 1, beta = 0."""
     import numpy as np
+    import torch
+
+    rng = np.random.default_rng(5)
+    cnt = 512
+    ms_ = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), cnt))).astype(int)
+    ns_ = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), cnt))).astype(int)
+    ks_ = np.rint(np.exp(rng.uniform(np.log(4096), np.log(65536), cnt))).astype(int)
+    offA = np.concatenate([[0], np.cumsum(ks_ * ms_)])
+    offB = np.concatenate([[0], np.cumsum(ks_ * ns_)])
+    offC = np.concatenate([[0], np.cumsum(ms_ * ns_)])
+    dA = uniform_device(SEED, 1, int(offA[-1]), True, dev)
+    dB = uniform_device(SEED, 2, int(offB[-1]), True, dev)
+    dC = torch.zeros(int(offC[-1]), dtype=torch.complex128, device=dev)
+    specs = []
+    full = tri = 0.0
+    for i in range(cnt):
+        mi, ni, ki = int(ms_[i]), int(ns_[i]), int(ks_[i])
+        fill = spla.FillMode.LOWER if i % 2 else spla.FillMode.FULL
+        specs.append(dict(m=mi, n=ni, k=ki, A=dA.data_ptr() + 16 * int(offA[i]), lda=ki,
+                          B=dB.data_ptr() + 16 * int(offB[i]), ldb=ki,
+                          C=dC.data_ptr() + 16 * int(offC[i]), ldc=mi, fill=fill))
+        full += 8.0 * mi * ni * ki
+        req = mi * ni if i % 2 == 0 else sum(min(ni, r + 1) for r in range(mi))
+        tri += 8.0 * req * ki
+    # accuracy: 8 x 8 sampled entries of every 16th block (required triangle only)
+    checks = []
+    nAB2 = 0.0
+    for i in range(0, cnt, 16):
+        mi, ni, ki = int(ms_[i]), int(ns_[i]), int(ks_[i])
+        rr, cc = sampled_rows_cols(mi, ni, 8, 100 + i)
+        Ai = dA[int(offA[i]):int(offA[i + 1])]
+        Bi = dB[int(offB[i]):int(offB[i + 1])]
+        checks.append((i, rr, cc, _ssb_reference(Ai, Bi, ki, mi, ni, rr, cc),
+                       torch.linalg.vector_norm(Ai).item() * torch.linalg.vector_norm(Bi).item()))
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    ctx.set_split_tolerance(tol)
+    out = {"config": {"workload": "512 complex ssb ring-step blocks A^H B through the batch API "
+                                  "(m, n log-uniform in [32, 512], k in [4096, 65536], seed 5; "
+                                  "odd blocks LOWER fill), alpha=1 beta=0",
+                      "flops_full_blocks": full, "flops_required_triangle": tri,
+                      "split_tolerance": tol},
+           "dtype": "c128", "forms": {}}
+    torch.cuda.synchronize()
+    for form in forms:
+        ctx.set_gemm_mode(_modes(spla)[form])
+        batch = spla.Batch(ctx, specs, 1.0, 0.0)
+        wall, ev = _timed(batch.execute, steps, warmup, stream)
+        worst = 0.0
+        for i, rr, cc, ref, nAB in checks:
+            mi, ni = int(ms_[i]), int(ns_[i])
+            got = _gather(dC[int(offC[i]):int(offC[i + 1])], mi, ni, rr, cc)
+            keep = torch.ones_like(ref.real, dtype=torch.bool)
+            if i % 2:  # LOWER: rows >= cols required
+                keep = torch.from_numpy(rr[:, None] >= cc[None, :]).to(dev)
+            d = (got - ref).abs()[keep]
+            err = float(torch.sqrt((d * d).mean() * (mi * ni)).item() / nAB)
+            worst = max(worst, err)
+        out["forms"][form] = {"tflops_full_blocks": round(full / wall / 1e9, 3),
+                              "tflops_required_triangle": round(tri / wall / 1e9, 3),
+                              "ms_per_step": round(wall, 3), "device_ms_per_step": round(ev, 3),
+                              "worst_block_normwise_error": worst, "blocks_checked": len(checks),
+                              "plan": batch.info()}
+        batch.close()
+        log(f"batch512 {form}: {out['forms'][form]}")
+    ctx.close()
+    del dA, dB, dC
+    _release()
+    return out
+
+
+def workload_ssb2048_c64(spla, dev, stream, steps, warmup, forms, tol):
+    """The headline shape in single precision: FP32 matrix cores, FP32 accumulation."""
+    import torch
+
+    m, n, k = M, N, K
+    dA = uniform_device(SEED, 1, k * m, True, dev).to(torch.complex64)
+    dB = uniform_device(SEED, 2, k * n, True, dev).to(torch.complex64)
+    dC = torch.zeros(m * n, dtype=torch.complex64, device=dev)
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 256,
+                                                             256)
+    flops = 8.0 * m * n * k
+    rows, cols = sampled_rows_cols(m, n, 32, 3)
+    ref = _ssb_reference(dA.to(torch.complex128), dB.to(torch.complex128), k, m, n, rows, cols)
+    nAB = (torch.linalg.vector_norm(dA.to(torch.complex128)).item() *
+           torch.linalg.vector_norm(dB.to(torch.complex128)).item())
+    run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, 1.0, dA, k, dB, k, 0.0,
+                                 dC, m, 0, 0, dist, ctx)
+    wall, ev = _timed(run, steps, warmup, stream)
+    got = _gather(dC, m, n, rows, cols).to(torch.complex128)
+    err = normwise_from_samples(got, ref, m, n, nAB)
+    out = {"config": {"workload": "spla_pcgemm_ssb A^H B complex float m=n=2048 k=200000, "
+                                  "device-resident, FP32 accumulation"},
+           "dtype": "c64",
+           "forms": {"plain": {"tflops": round(flops / wall / 1e9, 3),
+                               "ms_per_step": round(wall, 3), "device_ms_per_step": round(ev, 3),
+                               "normwise_error": err,
+                               "fraction_of_fp32_matrix_peak": round(flops / wall / 1e9 / 157.3, 4),
+                               "phase_ms": _phases(ctx, run)}}}
+    log(f"ssb2048_c64: {out['forms']['plain']}")
+    ctx.close()
+    del dA, dB, dC
+    _release()
+    return out
+
+
+def workload_ssb_small_mn(spla, dev, stream, steps, warmup, forms, tol):
+    """HBM-bound shapes: complex intensity m n / (2 (m + n)) FLOP/B is below the FP64
+    ridge 78.6 / 8 = 9.8 FLOP/B for m = n <= 39; plain FP64 kernels (split-K)."""
+    import torch
+
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_stream(stream)
+    ctx.set_gemm_mode(spla.GemmMode.FP64)
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 256,
+                                                             256)
+    out = {"config": {"workload": "spla_pzgemm_ssb A^H B complex, m = n = 16 (k = 8e6) and "
+                                  "32 (k = 4e6), device-resident, plain FP64, HBM-bound"},
+           "dtype": "f64", "forms": {}}
+    for (m, n, k) in ((16, 16, 8_000_000), (32, 32, 4_000_000)):
+        dA = uniform_device(SEED, 1, k * m, True, dev)
+        dB = uniform_device(SEED, 2, k * n, True, dev)
+        dC = uniform_device(SEED, 3, m * n, True, dev)
+        C0 = dC.clone()
+        flops = 8.0 * m * n * k
+        hbm = 16.0 * k * (m + n) + 2 * 16.0 * m * n  # A, B once; C read (beta != 0) and written
+        run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, dA, k, dB,
+                                     k, BETA, dC, m, 0, 0, dist, ctx)
+        dC.copy_(C0)
+        run()
+        rows, cols = sampled_rows_cols(m, n, 16, 5)
+        ref = ALPHA * _ssb_reference(dA, dB, k, m, n, rows, cols) + BETA * _gather(C0, m, n, rows,
+                                                                                   cols)
+        nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
+        err = normwise_from_samples(_gather(dC, m, n, rows, cols), ref, m, n, nAB)
+        wall, ev = _timed(run, steps, warmup, stream)
+        gbs = hbm / (wall / 1e3) / 1e9
+        key = f"m=n={m} k={k}"
+        out["forms"][key] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
+                             "device_ms_per_step": round(ev, 3), "hbm_gb_per_s": round(gbs, 1),
+                             "fraction_of_hbm_peak": round(gbs / 8000.0, 4),
+                             "intensity_flop_per_byte": round(flops / hbm, 2),
+                             "normwise_error": err, "phase_ms": _phases(ctx, run)}
+        log(f"ssb_small_mn {key}: {out['forms'][key]}")
+        del dA, dB, dC, C0
+        _release()
+    ctx.close()
+    return out
+
+
+WORKLOAD_FNS = {
+    "ssb2048_host": workload_ssb2048_host,
+    "sbs100k": workload_sbs100k,
+    "ssb_real_spread": workload_ssb_real_spread,
+    "batch512": workload_batch512,
+    "ssb2048_c64": workload_ssb2048_c64,
+    "ssb_small_mn": workload_ssb_small_mn,
+}
+
+
+def side_workload(args) -> int:
+    """One of the other parity workloads as its own JSON line (--workload NAME)."""
     import torch
 
     import spla_amd as spla
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
-    ctx = spla.Context(spla.ProcessingUnit.GPU)
-    ctx.set_stream(stream)
-    modes = {"plain": spla.GemmMode.FP64, "split": spla.GemmMode.SPLIT_INT8,
-             "auto": spla.GemmMode.AUTO}
-    steps, warmup = max(1, min(args.steps, 5)), max(1, min(args.warmup, 1))
+    steps, warmup = max(1, args.steps), max(0, args.warmup)
+    res = WORKLOAD_FNS[args.workload](spla, dev, stream, steps, warmup, args.forms.split(","),
+                                      args.split_tolerance)
     out = {"metric": METRIC, "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
-           "higher_is_better": True, "data": "synthetic (splitmix64 seed 42)", "forms": {}}
-    wl = args.workload
-
-    if wl == "sbs100k":
-        mL, n, k = 100000, 4096, 4096
-        dA = uniform_device(SEED, 1, mL * k, True, dev)
-        dB = uniform_device(SEED, 2, k * n, True, dev)
-        dC = torch.zeros(mL * n, dtype=torch.complex128, device=dev)
-        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
-                                                                 128, 128)
-        flops = 8.0 * mL * n * k
-        rows = np.random.default_rng(3).integers(0, mL, 32)
-        cols = np.random.default_rng(4).integers(0, n, 32)
-        a = dA.view(k, mL)[:, torch.from_numpy(rows).to(dev)].T.cpu().numpy()  # rows of A
-        b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy()
-        re = _accurate_dot(np.concatenate([a.real, -a.imag], -1), np.concatenate([b.real, b.imag], -1))
-        im = _accurate_dot(np.concatenate([a.real, a.imag], -1), np.concatenate([b.imag, b.real], -1))
-        ref = re + 1j * im
-        nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
-        for form in args.forms.split(","):
-            ctx.set_gemm_mode(modes[form])
-            run = lambda: spla.pgemm_sbs(mL, n, k, 1.0, dA, mL, dB, k, 0, 0, dist, 0.0, dC, mL, ctx)
-            wall, ev = _timed(run, steps, warmup, stream)
-            got = dC.view(n, mL).cpu().numpy()[cols, rows]
-            err = float(np.sqrt(np.mean(np.abs((got - ref) / nAB) ** 2) * mL * n))
-            info = ctx.last_product_info()
-            out["forms"][form] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
-                                  "device_ms_per_step": round(ev, 3), "normwise_error": err,
-                                  "split_used": info["split"], "moduli": info["moduli"],
-                                  "phase_ms": _phases(ctx, run)}
-            log(f"sbs100k {form}: {out['forms'][form]}")
-        out["config"] = {"workload": "spla_pzgemm_sbs complex, mLocal=100000 n=4096 k=4096, "
-                                     "device-resident, beta=0", "dist": "block cyclic 128x128 on 1x1"}
-        out["dtype"] = "c128"
-
-    elif wl == "ssb_real_spread":
-        m = n = 2048
-        k = 200000
-
-        def spread(mid, count, lo, hi):
-            x = uniform_device(SEED, mid, count, False, dev)
-            e = (uniform_device(SEED, mid + 100, count, False, dev) + 1.0) * 0.5 * (hi - lo + 1)
-            e = torch.clamp(torch.floor(e) + lo, lo, hi).to(torch.int32)
-            return torch.ldexp(x, e)
-
-        flops = 2.0 * m * n * k
-        rows = np.random.default_rng(3).integers(0, m, 32)
-        cols = np.random.default_rng(4).integers(0, n, 32)
-        for case, (lo, hi) in (("e in [-10, 10]", (-10, 10)), ("e in [-300, 300]", (-300, 300))):
-            dA = spread(11, k * m, lo, hi)
-            dB = spread(12, k * n, lo, hi)
-            dC = torch.zeros(m * n, dtype=torch.float64, device=dev)
-            dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
-                                                                     256, 256)
-            a = dA.view(m, k)[torch.from_numpy(rows).to(dev)].cpu().numpy()
-            b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy()
-            ref = _accurate_dot(a, b)
-            nAB = torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item()
-            for form in args.forms.split(","):
-                ctx.set_gemm_mode(modes[form])
-                run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.TRANSPOSE, 1.0, dA, k, dB, k,
-                                             0.0, dC, m, 0, 0, dist, ctx)
-                wall, ev = _timed(run, steps, warmup, stream)
-                got = dC.view(n, m).cpu().numpy()[cols, rows]
-                err = float(np.sqrt(np.mean(((got - ref) / nAB) ** 2) * m * n))
-                info = ctx.last_product_info()
-                out["forms"][f"{form}, {case}"] = {
-                    "tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
-                    "device_ms_per_step": round(ev, 3), "normwise_error": err,
-                    "split_used": info["split"], "fallback_reason": info["reason"],
-                    "moduli": info["moduli"], "bits": [info["bits_a"], info["bits_b"]],
-                    "phase_ms": _phases(ctx, run)}
-                log(f"ssb_real_spread {form} {case}: {out['forms'][f'{form}, {case}']}")
-            del dA, dB, dC
-        out["config"] = {"workload": "spla_pdgemm_ssb A^T B real, m=n=2048 k=200000, entries "
-                                     "uniform[-1,1] * 2^e", "dist": "block cyclic 256x256 on 1x1"}
-        out["dtype"] = "f64"
-
-    elif wl == "batch512":
-        rng = np.random.default_rng(5)
-        cnt = 512
-        ms_ = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), cnt))).astype(int)
-        ns_ = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), cnt))).astype(int)
-        ks_ = np.rint(np.exp(rng.uniform(np.log(4096), np.log(65536), cnt))).astype(int)
-        offA = np.concatenate([[0], np.cumsum(ks_ * ms_)])
-        offB = np.concatenate([[0], np.cumsum(ks_ * ns_)])
-        offC = np.concatenate([[0], np.cumsum(ms_ * ns_)])
-        dA = uniform_device(SEED, 1, int(offA[-1]), True, dev)
-        dB = uniform_device(SEED, 2, int(offB[-1]), True, dev)
-        dC = torch.zeros(int(offC[-1]), dtype=torch.complex128, device=dev)
-        specs = []
-        full = tri = 0.0
-        for i in range(cnt):
-            mi, ni, ki = int(ms_[i]), int(ns_[i]), int(ks_[i])
-            fill = spla.FillMode.LOWER if i % 2 else spla.FillMode.FULL
-            specs.append(dict(m=mi, n=ni, k=ki, A=dA.data_ptr() + 16 * int(offA[i]), lda=ki,
-                              B=dB.data_ptr() + 16 * int(offB[i]), ldb=ki,
-                              C=dC.data_ptr() + 16 * int(offC[i]), ldc=mi, fill=fill))
-            full += 8.0 * mi * ni * ki
-            req = mi * ni if i % 2 == 0 else sum(min(ni, r + 1) for r in range(mi))
-            tri += 8.0 * req * ki
-        torch.cuda.synchronize()
-        for form in args.forms.split(","):
-            ctx.set_gemm_mode(modes[form])
-            batch = spla.Batch(ctx, specs, 1.0, 0.0)
-            wall, ev = _timed(batch.execute, steps, warmup, stream)
-            out["forms"][form] = {"tflops_full_blocks": round(full / wall / 1e9, 3),
-                                  "tflops_required_triangle": round(tri / wall / 1e9, 3),
-                                  "ms_per_step": round(wall, 3), "device_ms_per_step": round(ev, 3),
-                                  "plan": batch.info()}
-            batch.close()
-            log(f"batch512 {form}: {out['forms'][form]}")
-        out["config"] = {"workload": "512 complex ssb ring-step blocks A^H B through the batch API "
-                                     "(m, n log-uniform in [32, 512], k in [4096, 65536], seed 5; "
-                                     "odd blocks LOWER fill), alpha=1 beta=0",
-                         "flops_full_blocks": full, "flops_required_triangle": tri}
-        out["dtype"] = "c128"
-
-    elif wl == "ssb2048_host":
-        m, n, k = M, N, K
-        hA = uniform_device(SEED, 1, k * m, True, dev).cpu().pin_memory()
-        hB = uniform_device(SEED, 2, k * n, True, dev).cpu().pin_memory()
-        hC = uniform_device(SEED, 3, m * n, True, dev).cpu().pin_memory()
-        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
-                                                                 256, 256)
-        flops = 8.0 * m * n * k
-        for form in args.forms.split(","):
-            ctx.set_gemm_mode(modes[form])
-            run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, hA, k, hB, k,
-                                         BETA, hC, m, 0, 0, dist, ctx)
-            wall, ev = _timed(run, steps, warmup, stream)
-            info = ctx.last_product_info()
-            out["forms"][form] = {"tflops_pcie_inclusive": round(flops / wall / 1e9, 3),
-                                  "ms_per_step": round(wall, 3), "split_used": info["split"]}
-            log(f"ssb2048_host {form}: {out['forms'][form]}")
-        out["config"] = {"workload": "spla_pzgemm_ssb A^H B complex m=n=2048 k=200000, A, B, C in "
-                                     "pinned host memory (PCIe-inclusive; not the reported value)"}
-        out["dtype"] = "c128"
-
-    elif wl == "ssb2048_c64":
-        # the headline shape in single precision: FP32 matrix cores, FP32 accumulation
-        m, n, k = M, N, K
-        dA = uniform_device(SEED, 1, k * m, True, dev).to(torch.complex64)
-        dB = uniform_device(SEED, 2, k * n, True, dev).to(torch.complex64)
-        dC = torch.zeros(m * n, dtype=torch.complex64, device=dev)
-        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
-                                                                 256, 256)
-        flops = 8.0 * m * n * k
-        rows = np.random.default_rng(3).integers(0, m, 32)
-        cols = np.random.default_rng(4).integers(0, n, 32)
-        a = dA.view(m, k)[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.complex128)
-        b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy().astype(np.complex128)
-        re = _accurate_dot(np.concatenate([a.real, a.imag], -1), np.concatenate([b.real, b.imag], -1))
-        im = _accurate_dot(np.concatenate([a.real, -a.imag], -1), np.concatenate([b.imag, b.real], -1))
-        ref = re + 1j * im  # conj(a) . b
-        nAB = (torch.linalg.vector_norm(dA.to(torch.complex128)).item() *
-               torch.linalg.vector_norm(dB.to(torch.complex128)).item())
-        ctx.set_gemm_mode(spla.GemmMode.FP64)
-        run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, 1.0, dA, k, dB, k,
-                                     0.0, dC, m, 0, 0, dist, ctx)
-        wall, ev = _timed(run, steps, warmup, stream)
-        got = dC.view(n, m).cpu().numpy().astype(np.complex128)[cols, rows]
-        err = float(np.sqrt(np.mean(np.abs((got - ref) / nAB) ** 2) * m * n))
-        out["forms"]["plain"] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
-                                 "device_ms_per_step": round(ev, 3), "normwise_error": err,
-                                 "fraction_of_fp32_matrix_peak": round(flops / wall / 1e9 / 157.3, 4),
-                                 "phase_ms": _phases(ctx, run)}
-        log(f"ssb2048_c64: {out['forms']['plain']}")
-        out["config"] = {"workload": "spla_pcgemm_ssb A^H B complex float m=n=2048 k=200000, "
-                                     "device-resident, FP32 accumulation"}
-        out["dtype"] = "c64"
-
-    elif wl == "ssb_small_mn":
-        # HBM-bound shapes: complex intensity m n / (2 (m + n)) FLOP/B is below the FP64
-        # ridge 78.6 / 8 = 9.8 FLOP/B for m = n <= 39; plain FP64 kernels (split-K)
-        dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1,
-                                                                 256, 256)
-        ctx.set_gemm_mode(spla.GemmMode.FP64)
-        for (m, n, k) in ((16, 16, 8_000_000), (32, 32, 4_000_000)):
-            dA = uniform_device(SEED, 1, k * m, True, dev)
-            dB = uniform_device(SEED, 2, k * n, True, dev)
-            dC = uniform_device(SEED, 3, m * n, True, dev)
-            C0 = dC.clone()
-            flops = 8.0 * m * n * k
-            hbm = 16.0 * k * (m + n) + 2 * 16.0 * m * n  # A, B once; C read (beta != 0) and written
-            run = lambda: spla.pgemm_ssb(m, n, k, spla.Operation.CONJ_TRANSPOSE, ALPHA, dA, k, dB,
-                                         k, BETA, dC, m, 0, 0, dist, ctx)
-            dC.copy_(C0)
-            run()
-            rows = np.random.default_rng(5).integers(0, m, 8)
-            cols = np.random.default_rng(6).integers(0, n, 8)
-            a = dA.view(m, k)[torch.from_numpy(rows).to(dev)].cpu().numpy()
-            b = dB.view(n, k)[torch.from_numpy(cols).to(dev)].cpu().numpy()
-            re = _accurate_dot(np.concatenate([a.real, a.imag], -1),
-                               np.concatenate([b.real, b.imag], -1))
-            im = _accurate_dot(np.concatenate([a.real, -a.imag], -1),
-                               np.concatenate([b.imag, b.real], -1))
-            c0 = C0.view(n, m).cpu().numpy()[cols, rows]
-            ref = ALPHA * (re + 1j * im) + BETA * c0
-            got = dC.view(n, m).cpu().numpy()[cols, rows]
-            nAB = (torch.linalg.vector_norm(dA).item() * torch.linalg.vector_norm(dB).item())
-            err = float(np.sqrt(np.mean(np.abs((got - ref) / nAB) ** 2) * m * n))
-            wall, ev = _timed(run, steps, warmup, stream)
-            gbs = hbm / (wall / 1e3) / 1e9
-            key = f"m=n={m} k={k}"
-            out["forms"][key] = {"tflops": round(flops / wall / 1e9, 3), "ms_per_step": round(wall, 3),
-                                 "device_ms_per_step": round(ev, 3),
-                                 "hbm_gb_per_s": round(gbs, 1),
-                                 "fraction_of_hbm_peak": round(gbs / 8000.0, 4),
-                                 "intensity_flop_per_byte": round(flops / hbm, 2),
-                                 "normwise_error": err, "phase_ms": _phases(ctx, run)}
-            log(f"ssb_small_mn {key}: {out['forms'][key]}")
-            del dA, dB, dC, C0
-        out["config"] = {"workload": "spla_pzgemm_ssb A^H B complex, m = n = 16 (k = 8e6) and "
-                                     "32 (k = 4e6), device-resident, plain FP64, HBM-bound"}
-        out["dtype"] = "f64"
-
+           "higher_is_better": True, "data": "synthetic (splitmix64 seed 42)", **res}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -498,6 +745,8 @@ def main() -> int:
                     help="normwise error bound the split form is planned for (the metric's "
                          "required accuracy; the library default is 1e-14)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the other BASELINE configurations after the headline")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (reference kernel for HBM counters)")
@@ -611,7 +860,9 @@ def main() -> int:
         roofline = {"bound": "mfma", "kernel": "i8_gemm_kernel (v_mfma_i32_32x32x32_i8)",
                     "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                     "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                    "traffic": HBM_TRAFFIC_BYTES["split"]}
+                    # counters were taken on the plan with PROFILED_MODULI moduli
+                    "traffic": (HBM_TRAFFIC_BYTES["split"] if best["moduli"] == PROFILED_MODULI
+                                else None)}
         dtype = "i8"
     else:
         gemm_ms = best["phase_ms_per_step"].get("gemm_fp64", best["device_ms_per_step"])
@@ -641,6 +892,26 @@ def main() -> int:
         "forms": forms,
         "roofline": roofline,
     }
+    ctx.close()
+    del dA, dB, dC0, dC
+    _release()
+    if not args.no_workloads:
+        # every other BASELINE configuration, measured in the same run (fewer steps)
+        line["workloads"] = {}
+        ws, ww = max(1, min(args.steps, 3)), 1
+        for name in DEFAULT_SIDE:
+            log(f"workload {name}")
+            t0 = time.perf_counter()
+            try:
+                res = WORKLOAD_FNS[name](spla, dev, stream, ws, ww, SIDE_FORMS[name],
+                                         args.split_tolerance)
+            except Exception as exc:  # recorded, the headline line is still printed
+                res = {"error": f"{type(exc).__name__}: {exc}"}
+                log(f"workload {name} failed: {res['error']}")
+                _release()
+            res["steps"], res["warmup"] = ws, ww
+            res["wall_s"] = round(time.perf_counter() - t0, 1)
+            line["workloads"][name] = res
     if not args.no_cpu_baseline:
         log("CPU baseline (host BLAS, k-truncated sample)")
         line["cpu_baseline"] = cpu_baseline(m, n, k)

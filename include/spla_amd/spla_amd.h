@@ -86,18 +86,20 @@ SPLA_EXPORT SplaError spla_amd_ctx_last_product_info(SplaContext ctx, int* used,
  * One ring step of pgemm_ssb / pgemm_ssbtr forms many independent block
  * products C_b <- alpha op(A_b) B_b + beta C_b with op(A) = A^T or A^H
  * (SPLA src/pgemm_ssb/ring_ssb_gpu.cpp:193-197 issues one vendor GEMM per
- * block). A batch is planned for one persistent launch (tile shape and
- * per-block split-K factors from a cost model of the persistent schedule);
- * in this build that launch is switched off and the blocks run one after the
- * other on the single-product engine, with the same results. All pointers
- * must be device memory. A block's fill mode and offsets follow pgemm_ssbtr:
- * tiles wholly outside the requested triangle of the global C are skipped.
+ * block); one of pgemm_sbs forms C(:, cols) <- alpha A(:, rows) B_blk +
+ * beta' C(:, cols) (src/pgemm_sbs/ring_sbs_gpu.cpp:57-88, 278-282). A batch
+ * is planned for one persistent launch (tile shape and per-block split-K
+ * factors from a cost model of the persistent schedule); in this build that
+ * launch is switched off and the blocks run one after the other on the
+ * single-product engine, with the same results. All pointers must be device
+ * memory. A block's fill mode and offsets follow pgemm_ssbtr: tiles wholly
+ * outside the requested triangle of the global C are skipped.
  */
 typedef void* SplaAmdBatch;
 
 typedef struct SplaAmdBlock {
   int m, n, k;        /* C block m x n, reduction length k */
-  const void* A;      /* k x m, leading dimension lda >= k */
+  const void* A;      /* ssb: k x m, lda >= k; sbs: m x k, lda >= m */
   int lda;
   const void* B;      /* k x n, leading dimension ldb >= k */
   int ldb;
@@ -125,6 +127,15 @@ typedef enum SplaAmdTuning SplaAmdTuning;
  * (re, im) pair. The block array is copied. */
 SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext ctx,
                                                 SplaAmdElement type, SplaOperation opA, int count,
+                                                const SplaAmdBlock* blocks, const void* alpha,
+                                                const void* beta, SplaAmdTuning tuning);
+
+/* The same for the block products of a pgemm_sbs ring step: op(A) = A with A
+ * m x k (lda >= m), B k x n, every block's fill mode SPLA_FILL_MODE_FULL. A
+ * ring step applies beta only on the first visit of a column block
+ * (ring_sbs_gpu.cpp:278-282): products with beta and with 1 are two batches. */
+SPLA_EXPORT SplaError spla_amd_batch_create_sbs(SplaAmdBatch* batch, SplaContext ctx,
+                                                SplaAmdElement type, int count,
                                                 const SplaAmdBlock* blocks, const void* alpha,
                                                 const void* beta, SplaAmdTuning tuning);
 

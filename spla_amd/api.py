@@ -327,7 +327,8 @@ class Batch:
 
     blocks: sequence of dicts with keys m, n, k, A, lda, B, ldb, C, ldc and optional
     fill (FillMode), row_offset, col_offset; arrays are device tensors or raw
-    device addresses.
+    device addresses. opA TRANSPOSE / CONJ_TRANSPOSE: pgemm_ssb blocks (A k x m);
+    opA NONE: pgemm_sbs blocks (A m x k, spla_amd_batch_create_sbs).
     """
 
     def __init__(self, ctx: Context, blocks, alpha, beta, complex_: bool = True,
@@ -344,9 +345,14 @@ class Batch:
         self._alpha = np.array([alpha], dtype=dt)
         self._beta = np.array([beta], dtype=dt)
         self._h = ctypes.c_void_p()
-        check(self._lib.spla_amd_batch_create_ssb(
-            ctypes.byref(self._h), ctx.handle, 1 if complex_ else 0, int(opA), len(blocks), arr,
-            self._alpha.ctypes.data, self._beta.ctypes.data, 1 if measure else 0))
+        if int(opA) == int(Operation.NONE):
+            check(self._lib.spla_amd_batch_create_sbs(
+                ctypes.byref(self._h), ctx.handle, 1 if complex_ else 0, len(blocks), arr,
+                self._alpha.ctypes.data, self._beta.ctypes.data, 1 if measure else 0))
+        else:
+            check(self._lib.spla_amd_batch_create_ssb(
+                ctypes.byref(self._h), ctx.handle, 1 if complex_ else 0, int(opA), len(blocks),
+                arr, self._alpha.ctypes.data, self._beta.ctypes.data, 1 if measure else 0))
 
     def execute(self) -> None:
         check(self._lib.spla_amd_batch_execute(self._h))

@@ -97,6 +97,14 @@ SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext
   })
 }
 
+SPLA_EXPORT SplaError spla_amd_batch_create_sbs(SplaAmdBatch* batch, SplaContext ctx,
+                                                SplaAmdElement type, int count,
+                                                const SplaAmdBlock* blocks, const void* alpha,
+                                                const void* beta, SplaAmdTuning tuning) {
+  return spla_amd_batch_create_ssb(batch, ctx, type, SPLA_OP_NONE, count, blocks, alpha, beta,
+                                   tuning);
+}
+
 SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch) {
   if (!batch) return SPLA_INVALID_HANDLE_ERROR;
   SPLA_C_GUARD(static_cast<spla_amd::BatchPlan*>(batch)->execute())

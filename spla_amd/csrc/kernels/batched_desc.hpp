@@ -1,6 +1,8 @@
-// Descriptors of the persistent batched kernel (batched.hip): many independent
-// block products C_b <- alpha op(A_b) B_b + beta C_b (ring-step block products
-// of pgemm_ssb / pgemm_ssbtr) in one launch.
+// Descriptors of the batched block-product kernels (batched.hip): many
+// independent block products C_b <- alpha op(A_b) B_b + beta C_b (the block
+// products of one pgemm_ssb / pgemm_ssbtr or pgemm_sbs ring step) in one
+// grouped GEMM launch over the tiles of every block, followed by one
+// fixed-order reduction launch over the tiles whose reduction is split.
 #pragma once
 
 #include <cstdint>
@@ -13,27 +15,28 @@ struct BatchBlock {
   double* C;
   int64_t lda, ldb, ldc;  // elements
   int m, n;               // block of C (elements)
-  int64_t Kp;             // embedded reduction length (k or 2k)
+  int64_t Kp;             // embedded reduction length (k or 2k; 0: C <- beta C only)
   int Mp;                 // embedded rows (m or 2m)
-  int tilesN;
-  int splits;
-  int64_t kSplit;         // embedded reduction length per split
+  int splits;             // reduction splits of every tile of the block
+  int64_t kSplit;         // embedded reduction length per split (whole k-blocks)
   int64_t workOffset;     // doubles: this block's [splits][n][Mp] partial sums
-  int counterBase;        // first of the block's tilesM * tilesN arrival counters
 };
 
-// One work item: (block, tile row, tile column, split).
+// One workgroup of the GEMM launch: (block, tile row, tile column, split).
 struct BatchItem {
   int block, tm, tn, s;
 };
 
+// One workgroup of the reduction launch: a tile of a block with splits > 1.
+struct BatchReduceTile {
+  int block, tm, tn;
+};
+
 struct BatchDesc {
   const BatchBlock* blocks;
-  const BatchItem* items;
-  int numItems;
+  const BatchItem* items;          // gridDim.x of the GEMM launch
+  const BatchReduceTile* reduce;   // gridDim.x of the reduction launch
   double* work;
-  unsigned* counters;  // per split tile; every launch leaves them at 0
-  unsigned* queue;     // [0] next item, [1] finished workgroups; left at 0
   double alphaRe, alphaIm, betaRe, betaIm;
   int betaZero;
   int conjA, conjB;

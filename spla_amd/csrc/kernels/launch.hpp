@@ -19,6 +19,13 @@ hipError_t launch_gemm_f64(const GemmDesc& d, bool complex, int layoutA, int lay
 hipError_t launch_gemm_f32(const GemmDesc& d, bool complex, int layoutA, int layoutB, int cfg,
                            hipStream_t stream);
 
+// Streaming FP64 / complex-FP64 product for small m, n with K-major A and B
+// (skinny.hip): one workgroup per split (gridDim = d.splits >= 2), each summing
+// its d.kSplit embedded reduction indices into d.work ([splits][n][Mp]) for
+// launch_splitk_reduce.
+bool skinny_supported(bool complex, int layoutA, int layoutB, int64_t m, int64_t n);
+hipError_t launch_skinny_gemm(const GemmDesc& d, bool complex, hipStream_t stream);
+
 // Fixed-order sum of d.splits partial results in d.work, then alpha / beta
 // (f32: the scalars of d are floats).
 hipError_t launch_splitk_reduce(const GemmDesc& d, bool complex, int cfg, hipStream_t stream,
@@ -38,8 +45,16 @@ hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream);
 // limbs: CrtConstants::limbs of the constants d.cc points to
 hipError_t launch_crt(bool complex, const CrtDesc& d, int limbs, hipStream_t stream);
 
-// Persistent batched block products (batched.hip); cfg is TileL or TileM.
-hipError_t launch_batched_ssb(const BatchDesc& d, bool complex, int cfg, int workgroups,
-                              hipStream_t stream);
+// Batched block products (batched.hip): the grouped GEMM kernel (one workgroup
+// per BatchItem, kernel argument one BatchDesc) and the split reduction kernel
+// (one workgroup per BatchReduceTile) of a tile configuration (TileL or TileM)
+// and A layout (KMajor: ssb blocks, MNMajor: sbs blocks), as kernel handles for
+// hipLaunchKernel and hipGraphAddKernelNode; `threads` per workgroup.
+struct BatchKernels {
+  const void* gemm;
+  const void* reduce;
+  int threads;
+};
+BatchKernels batched_kernels(bool complex, int layoutA, int cfg);
 
 }  // namespace spla_amd

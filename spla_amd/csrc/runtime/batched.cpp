@@ -10,6 +10,7 @@
 
 #include "kernels/gemm_desc.hpp"
 #include "kernels/launch.hpp"
+#include "runtime/batch_schedule.hpp"
 #include "runtime/engine.hpp"
 #include "runtime/hip_check.hpp"
 #include "runtime/split_gemm.hpp"
@@ -17,28 +18,6 @@
 
 namespace spla_amd {
 namespace {
-
-constexpr double kPeakPerCU = 128.0 * 2.4e9;  // FP64 matrix flop / s per CU
-constexpr double kItemOverhead = 2.0e-6;      // scheduling + epilogue per item
-constexpr double kLaunchOverhead = 8.0e-6;
-
-// The persistent launch of batched.hip (one hipGraph node per batch) is
-// switched off: its first run on an MI355X produced no result within three
-// minutes and the cause has not been identified. Until it is, execute() runs
-// the blocks one by one on the single-product engine (same kernels as gemm).
-constexpr bool kPersistentLaunch = false;
-constexpr double kWorkgroupBytesPerSecond = 60e9;  // one workgroup reducing split partials
-
-int blocks_per_cu(int cfg) { return cfg == TileM ? 4 : 2; }
-double efficiency(int cfg) { return cfg == TileM ? 0.55 : 0.75; }
-
-bool tile_active(int mode, int64_t rowOff, int64_t colOff, int64_t r, int64_t c, int64_t nr,
-                 int64_t nc) {
-  // SPLA block_is_active (src/block_generation/block.hpp:58-66).
-  if (mode == SPLA_FILL_MODE_UPPER) return c + nc + colOff > r + rowOff;
-  if (mode == SPLA_FILL_MODE_LOWER) return r + nr + rowOff > c + colOff;
-  return true;
-}
 
 class DeviceGuard {
 public:
@@ -148,12 +127,116 @@ void execute_split_batch(spla::ContextInternal& ctx, const std::vector<BlockSpec
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// GroupedLaunch
+
+GroupedLaunch::~GroupedLaunch() { release_graph(); }
+
+void GroupedLaunch::release_graph() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void GroupedLaunch::build(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
+                          const std::vector<BlockSpec>& specs, const double* alpha,
+                          const double* beta, const BatchSchedule& sched, bool withGraph) {
+  release_graph();
+  const hipStream_t s = ctx.compute_stream();
+  std::vector<BatchBlock> blocks = sched.blocks;
+  for (std::size_t b = 0; b < specs.size(); ++b) {
+    const BlockSpec& sp = specs[b];
+    BatchBlock& bb = blocks[b];
+    bb.A = static_cast<const double*>(sp.A);
+    bb.B = static_cast<const double*>(sp.B);
+    bb.C = static_cast<double*>(sp.C);
+    bb.lda = sp.lda;
+    bb.ldb = sp.ldb;
+    bb.ldc = sp.ldc;
+  }
+  dBlocks_ = Block(ctx.gpu_memory(), std::max<std::size_t>(1, blocks.size()) * sizeof(BatchBlock));
+  dItems_ = Block(ctx.gpu_memory(), std::max<std::size_t>(1, sched.items.size()) * sizeof(BatchItem));
+  dReduce_ = Block(ctx.gpu_memory(),
+                   std::max<std::size_t>(1, sched.reduce.size()) * sizeof(BatchReduceTile));
+  dWork_ = Block(ctx.gpu_memory(), std::max<int64_t>(1, sched.workDoubles) * sizeof(double));
+  if (!blocks.empty())
+    hip_check(hipMemcpyAsync(dBlocks_.get(), blocks.data(), blocks.size() * sizeof(BatchBlock),
+                             hipMemcpyHostToDevice, s));
+  if (!sched.items.empty())
+    hip_check(hipMemcpyAsync(dItems_.get(), sched.items.data(),
+                             sched.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, s));
+  if (!sched.reduce.empty())
+    hip_check(hipMemcpyAsync(dReduce_.get(), sched.reduce.data(),
+                             sched.reduce.size() * sizeof(BatchReduceTile), hipMemcpyHostToDevice,
+                             s));
+  hip_check(hipStreamSynchronize(s));
+  std::memset(&desc_, 0, sizeof(desc_));
+  desc_.blocks = dBlocks_.get<BatchBlock>();
+  desc_.items = dItems_.get<BatchItem>();
+  desc_.reduce = dReduce_.get<BatchReduceTile>();
+  desc_.work = dWork_.get<double>();
+  desc_.alphaRe = alpha[0];
+  desc_.alphaIm = complex ? alpha[1] : 0.0;
+  desc_.betaRe = beta[0];
+  desc_.betaIm = complex ? beta[1] : 0.0;
+  desc_.betaZero = (desc_.betaRe == 0.0 && desc_.betaIm == 0.0) ? 1 : 0;
+  desc_.conjA = (complex && opA == SPLA_OP_CONJ_TRANSPOSE) ? 1 : 0;
+  desc_.conjB = 0;
+  kernels_ = batched_kernels(complex, opA == SPLA_OP_NONE ? MNMajor : KMajor, sched.cfg);
+  numItems_ = static_cast<unsigned>(sched.items.size());
+  numReduce_ = static_cast<unsigned>(sched.reduce.size());
+  if (withGraph && numItems_ > 0) {
+    // the two launches as the nodes of one graph, replayed by run()
+    hip_check(hipGraphCreate(&graph_, 0));
+    void* args[] = {&desc_};
+    hipKernelNodeParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.func = const_cast<void*>(kernels_.gemm);
+    p.gridDim = dim3(numItems_);
+    p.blockDim = dim3(kernels_.threads);
+    p.sharedMemBytes = 0;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    hipGraphNode_t gemmNode = nullptr;
+    hip_check(hipGraphAddKernelNode(&gemmNode, graph_, nullptr, 0, &p));
+    if (numReduce_ > 0) {
+      p.func = const_cast<void*>(kernels_.reduce);
+      p.gridDim = dim3(numReduce_);
+      hipGraphNode_t reduceNode = nullptr;
+      hip_check(hipGraphAddKernelNode(&reduceNode, graph_, &gemmNode, 1, &p));
+    }
+    hip_check(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+}
+
+void GroupedLaunch::launch(hipStream_t s) {
+  void* args[] = {&desc_};
+  if (numItems_ > 0)
+    hip_check(hipLaunchKernel(kernels_.gemm, dim3(numItems_), dim3(kernels_.threads), args, 0, s));
+  if (numReduce_ > 0)
+    hip_check(
+        hipLaunchKernel(kernels_.reduce, dim3(numReduce_), dim3(kernels_.threads), args, 0, s));
+}
+
+void GroupedLaunch::run(hipStream_t s) {
+  if (exec_)
+    hip_check(hipGraphLaunch(exec_, s));
+  else
+    launch(s);
+}
+
+// ---------------------------------------------------------------------------
+// BatchPlan
+
 BatchPlan::BatchPlan(std::shared_ptr<spla::ContextInternal> ctx, bool complex, SplaOperation opA,
                      std::vector<BlockSpec> blocks, const double* alpha, const double* beta,
                      bool measure)
     : ctx_(std::move(ctx)), complex_(complex), opA_(opA), specs_(std::move(blocks)) {
-  if (opA != SPLA_OP_TRANSPOSE && opA != SPLA_OP_CONJ_TRANSPOSE)
+  // ssb blocks: op(A) = A^T or A^H (A k x m); sbs blocks: op(A) = A (A m x k, full fill)
+  if (opA != SPLA_OP_TRANSPOSE && opA != SPLA_OP_CONJ_TRANSPOSE && opA != SPLA_OP_NONE)
     throw spla::InvalidParameterError();
+  const bool sbs = opA == SPLA_OP_NONE;
   alpha_[0] = alpha[0];
   alpha_[1] = complex ? alpha[1] : 0.0;
   beta_[0] = beta[0];
@@ -162,11 +245,11 @@ BatchPlan::BatchPlan(std::shared_ptr<spla::ContextInternal> ctx, bool complex, S
   for (const BlockSpec& b : specs_) {
     if (b.m < 0 || b.n < 0 || b.k < 0 || b.rowOffset < 0 || b.colOffset < 0)
       throw spla::InvalidParameterError();
-    if (b.fillMode != SPLA_FILL_MODE_FULL && b.fillMode != SPLA_FILL_MODE_UPPER &&
-        b.fillMode != SPLA_FILL_MODE_LOWER)
+    if (b.fillMode != SPLA_FILL_MODE_FULL &&
+        (sbs || (b.fillMode != SPLA_FILL_MODE_UPPER && b.fillMode != SPLA_FILL_MODE_LOWER)))
       throw spla::InvalidParameterError();
     if (b.m == 0 || b.n == 0) continue;
-    if (b.lda < std::max(1, b.k) || b.ldb < std::max(1, b.k) || b.ldc < b.m)
+    if (b.lda < std::max(1, sbs ? b.m : b.k) || b.ldb < std::max(1, b.k) || b.ldc < b.m)
       throw spla::InvalidParameterError();
     if ((b.k > 0 && (!b.A || !b.B)) || !b.C) throw spla::InvalidPointerError();
   }
@@ -176,183 +259,51 @@ BatchPlan::BatchPlan(std::shared_ptr<spla::ContextInternal> ctx, bool complex, S
     if ((b.k > 0 && (!on_device(b.A) || !on_device(b.B))) || !on_device(b.C))
       throw spla::GPUInvalidDevicePointerError();
   }
-  BatchCandidate best = build(TileL, 1.0);
-  BatchCandidate other = build(TileM, 1.0);
-  if (other.modelSeconds < best.modelSeconds) std::swap(best, other);
-  if (!kPersistentLaunch) {
-    // switched off: the blocks run one by one on the single-product engine
-    cand_ = std::move(best);
-    return;
-  }
+  // Autotuning: the model picks the tile shape (and scales the split counts);
+  // in measured mode every candidate also runs on scratch copies of C, the
+  // fastest is kept and its measured rate corrects the model's efficiency for
+  // the batches the context plans afterwards.
+  BatchSchedule best = best_schedule(specs_);
   if (measure && !best.items.empty()) {
-    std::vector<BatchCandidate> cands;
-    for (int cfg : {TileL, TileM})
-      for (double sc : {0.5, 1.0, 2.0}) cands.push_back(build(cfg, sc));
     double bestMs = 1e300;
-    for (BatchCandidate& c : cands) {
-      const double ms = time_candidate(c);
-      if (ms < bestMs) {
-        bestMs = ms;
-        best = c;
+    for (int cfg : {TileL, TileM})
+      for (double sc : {0.5, 1.0, 2.0}) {
+        BatchSchedule c = schedule(specs_, cfg, sc);
+        const double ms = time_candidate(c);
+        if (ms < bestMs) {
+          bestMs = ms;
+          best = std::move(c);
+        }
       }
-    }
     measuredMs_ = bestMs;
+    const double eff = ctx_->batch_efficiency(best.cfg) * best.modelSeconds / (bestMs * 1e-3);
+    ctx_->set_batch_efficiency(best.cfg, std::min(0.95, std::max(0.05, eff)));
   }
-  cand_ = std::move(best);
-  upload(cand_);
-  capture();
+  sched_ = std::move(best);
+  full_.build(*ctx_, complex_, opA_, specs_, alpha_, beta_, sched_, /*withGraph=*/true);
 }
 
-BatchPlan::~BatchPlan() {
-  if (exec_) (void)hipGraphExecDestroy(exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
+BatchPlan::~BatchPlan() = default;
+
+BatchSchedule BatchPlan::schedule(const std::vector<BlockSpec>& specs, int cfg,
+                                  double splitScale) const {
+  std::vector<BatchShape> shapes;
+  shapes.reserve(specs.size());
+  for (const BlockSpec& b : specs)
+    shapes.push_back({b.m, b.n, b.k, b.fillMode, b.rowOffset, b.colOffset});
+  const bool alphaZero = alpha_[0] == 0.0 && alpha_[1] == 0.0;
+  return plan_batch_schedule(shapes, complex_, alphaZero, cfg, splitScale, ctx_->num_cus(),
+                             ctx_->batch_efficiency(cfg));
 }
 
-BatchCandidate BatchPlan::build(int cfg, double splitScale) const {
-  BatchCandidate c;
-  c.cfg = cfg;
-  c.splitScale = splitScale;
-  const TileShape ts = tile_shape(cfg);
-  const int comps = complex_ ? 2 : 1;
-  const int ar = ts.bm / comps;  // element rows per tile
-  const int slots = ctx_->num_cus() * blocks_per_cu(cfg);
-  const double slotRate = kPeakPerCU * efficiency(cfg) / blocks_per_cu(cfg);
-
-  struct Tile {
-    int block, tm, tn;
-    double cost;
-  };
-  std::vector<Tile> tiles;
-  std::vector<int> tilesM(specs_.size(), 0), tilesN(specs_.size(), 0);
-  double total = 0.0;
-  for (std::size_t b = 0; b < specs_.size(); ++b) {
-    const BlockSpec& s = specs_[b];
-    if (s.m == 0 || s.n == 0) continue;
-    const int64_t Kp = static_cast<int64_t>(s.k) * comps;
-    tilesM[b] = (s.m + ar - 1) / ar;
-    tilesN[b] = (s.n + ts.bn - 1) / ts.bn;
-    const double cost = 2.0 * ts.bm * ts.bn * static_cast<double>(std::max<int64_t>(Kp, ts.bk)) /
-                        slotRate;
-    for (int tm = 0; tm < tilesM[b]; ++tm)
-      for (int tn = 0; tn < tilesN[b]; ++tn) {
-        const int64_t r = static_cast<int64_t>(tm) * ar, col = static_cast<int64_t>(tn) * ts.bn;
-        if (!tile_active(s.fillMode, s.rowOffset, s.colOffset, r, col,
-                         std::min<int64_t>(ar, s.m - r), std::min<int64_t>(ts.bn, s.n - col)))
-          continue;
-        tiles.push_back({static_cast<int>(b), tm, tn, cost});
-        total += cost;
-      }
-  }
-  // target cost of one item: at least three items per slot, at least ~20 us
-  const double target = std::max(total / (3.0 * std::max(1, slots)), 20e-6);
-
-  c.blocks.resize(specs_.size());
-  std::vector<int> splits(specs_.size(), 1);
-  for (std::size_t b = 0; b < specs_.size(); ++b) {
-    const BlockSpec& s = specs_[b];
-    BatchBlock& bb = c.blocks[b];
-    std::memset(&bb, 0, sizeof(bb));
-    if (s.m == 0 || s.n == 0) continue;
-    const int64_t Kp = static_cast<int64_t>(s.k) * comps;
-    const int64_t kBlocks = (Kp + ts.bk - 1) / ts.bk;
-    const double tileCost = 2.0 * ts.bm * ts.bn * static_cast<double>(std::max<int64_t>(Kp, 1)) /
-                            slotRate;
-    int64_t sp = static_cast<int64_t>(std::ceil(tileCost * splitScale / target));
-    sp = std::max<int64_t>(1, std::min<int64_t>(sp, std::max<int64_t>(1, kBlocks / 16)));
-    const int64_t kbPer = kBlocks > 0 ? (kBlocks + sp - 1) / sp : 1;
-    sp = kBlocks > 0 ? (kBlocks + kbPer - 1) / kbPer : 1;
-    splits[b] = static_cast<int>(sp);
-    bb.A = static_cast<const double*>(s.A);
-    bb.B = static_cast<const double*>(s.B);
-    bb.C = static_cast<double*>(s.C);
-    bb.lda = s.lda;
-    bb.ldb = s.ldb;
-    bb.ldc = s.ldc;
-    bb.m = s.m;
-    bb.n = s.n;
-    bb.Kp = Kp;
-    bb.Mp = s.m * comps;
-    bb.tilesN = tilesN[b];
-    bb.splits = static_cast<int>(sp);
-    bb.kSplit = kbPer * ts.bk;
-    if (sp > 1) {
-      bb.workOffset = c.workDoubles;
-      c.workDoubles += sp * static_cast<int64_t>(s.n) * bb.Mp;
-      bb.counterBase = c.counters;
-      c.counters += tilesM[b] * tilesN[b];
-    }
-  }
-  // items in decreasing cost order; LPT simulation of the persistent schedule
-  struct Costed {
-    BatchItem item;
-    double cost;
-  };
-  std::vector<Costed> items;
-  for (const Tile& t : tiles) {
-    const int sp = splits[t.block];
-    // the last split of a tile also adds up the partial sums of all splits
-    const double reduce =
-        sp > 1 ? (sp + 2.0) * ar * ts.bn * 8.0 * comps / kWorkgroupBytesPerSecond : 0.0;
-    for (int q = 0; q < sp; ++q)
-      items.push_back({{t.block, t.tm, t.tn, q}, t.cost / sp + kItemOverhead + reduce / sp});
-  }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const Costed& a, const Costed& b) { return a.cost > b.cost; });
-  c.items.reserve(items.size());
-  for (const Costed& it : items) c.items.push_back(it.item);
-  c.workgroups = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(slots, items.size())));
-  std::priority_queue<double, std::vector<double>, std::greater<double>> load;
-  for (int i = 0; i < c.workgroups; ++i) load.push(0.0);
-  double makespan = 0.0;
-  for (const Costed& it : items) {
-    double l = load.top();
-    load.pop();
-    l += it.cost;
-    makespan = std::max(makespan, l);
-    load.push(l);
-  }
-  c.modelSeconds = makespan + kLaunchOverhead;
-  return c;
+BatchSchedule BatchPlan::best_schedule(const std::vector<BlockSpec>& specs) const {
+  BatchSchedule best = schedule(specs, TileL, 1.0);
+  BatchSchedule other = schedule(specs, TileM, 1.0);
+  if (other.modelSeconds < best.modelSeconds) std::swap(best, other);
+  return best;
 }
 
-void BatchPlan::upload(const BatchCandidate& c, std::vector<BatchBlock>* override) {
-  const hipStream_t s = ctx_->compute_stream();
-  const std::vector<BatchBlock>& blocks = override ? *override : c.blocks;
-  dBlocks_ = Block(ctx_->gpu_memory(), std::max<std::size_t>(1, blocks.size()) * sizeof(BatchBlock));
-  dItems_ = Block(ctx_->gpu_memory(), std::max<std::size_t>(1, c.items.size()) * sizeof(BatchItem));
-  dWork_ = Block(ctx_->gpu_memory(), std::max<int64_t>(1, c.workDoubles) * sizeof(double));
-  dCounters_ = Block(ctx_->gpu_memory(), (c.counters + 2) * sizeof(unsigned));
-  if (!blocks.empty())
-    hip_check(hipMemcpyAsync(dBlocks_.get(), blocks.data(), blocks.size() * sizeof(BatchBlock),
-                             hipMemcpyHostToDevice, s));
-  if (!c.items.empty())
-    hip_check(hipMemcpyAsync(dItems_.get(), c.items.data(), c.items.size() * sizeof(BatchItem),
-                             hipMemcpyHostToDevice, s));
-  // counters and queue start at zero; every launch leaves them at zero
-  hip_check(hipMemsetAsync(dCounters_.get(), 0, dCounters_.bytes(), s));
-  hip_check(hipStreamSynchronize(s));
-  std::memset(&desc_, 0, sizeof(desc_));
-  desc_.blocks = dBlocks_.get<BatchBlock>();
-  desc_.items = dItems_.get<BatchItem>();
-  desc_.numItems = static_cast<int>(c.items.size());
-  desc_.work = dWork_.get<double>();
-  desc_.counters = dCounters_.get<unsigned>() + 2;
-  desc_.queue = dCounters_.get<unsigned>();
-  desc_.alphaRe = alpha_[0];
-  desc_.alphaIm = alpha_[1];
-  desc_.betaRe = beta_[0];
-  desc_.betaIm = beta_[1];
-  desc_.betaZero = (beta_[0] == 0.0 && beta_[1] == 0.0) ? 1 : 0;
-  desc_.conjA = (complex_ && opA_ == SPLA_OP_CONJ_TRANSPOSE) ? 1 : 0;
-  desc_.conjB = 0;
-}
-
-void BatchPlan::launch(hipStream_t s) {
-  if (desc_.numItems == 0) return;
-  hip_check(launch_batched_ssb(desc_, complex_, cand_.cfg, cand_.workgroups, s));
-}
-
-double BatchPlan::time_candidate(const BatchCandidate& c) {
+double BatchPlan::time_candidate(const BatchSchedule& c) {
   // Run the candidate on scratch copies of C (C itself is left untouched).
   const int comps = complex_ ? 2 : 1;
   int64_t total = 0;
@@ -360,44 +311,29 @@ double BatchPlan::time_candidate(const BatchCandidate& c) {
   Block scratch(ctx_->gpu_memory(), std::max<int64_t>(1, total) * sizeof(double));
   const hipStream_t st = ctx_->compute_stream();
   hip_check(hipMemsetAsync(scratch.get(), 0, scratch.bytes(), st));
-  std::vector<BatchBlock> shadow = c.blocks;
+  std::vector<BlockSpec> shadow = specs_;
   int64_t off = 0;
-  for (std::size_t b = 0; b < specs_.size(); ++b) {
-    shadow[b].C = scratch.get<double>() + off;
-    shadow[b].ldc = specs_[b].m;
-    off += static_cast<int64_t>(specs_[b].m) * specs_[b].n * comps;
+  for (BlockSpec& b : shadow) {
+    b.C = scratch.get<double>() + off;
+    b.ldc = std::max(1, b.m);
+    off += static_cast<int64_t>(b.m) * b.n * comps;
   }
-  const BatchCandidate saved = cand_;
-  cand_ = c;
-  upload(c, &shadow);
+  GroupedLaunch trial;
+  trial.build(*ctx_, complex_, opA_, shadow, alpha_, beta_, c, /*withGraph=*/false);
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0));
   hip_check(hipEventCreate(&e1));
-  launch(st);  // warm-up
+  trial.launch(st);  // warm-up
   hip_check(hipEventRecord(e0, st));
   constexpr int kReps = 3;
-  for (int r = 0; r < kReps; ++r) launch(st);
+  for (int r = 0; r < kReps; ++r) trial.launch(st);
   hip_check(hipEventRecord(e1, st));
   hip_check(hipEventSynchronize(e1));
   float ms = 0.0f;
   hip_check(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  cand_ = saved;
   return ms / kReps;
-}
-
-void BatchPlan::capture() {
-  if (desc_.numItems == 0) return;
-  const hipStream_t s = ctx_->compute_stream();
-  hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  const hipError_t launched = launch_batched_ssb(desc_, complex_, cand_.cfg, cand_.workgroups, s);
-  hipGraph_t g = nullptr;
-  const hipError_t ended = hipStreamEndCapture(s, &g);
-  hip_check(launched);
-  hip_check(ended);
-  graph_ = g;
-  hip_check(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
 }
 
 void BatchPlan::execute() {
@@ -405,16 +341,44 @@ void BatchPlan::execute() {
   hip_check(hipStreamSynchronize(nullptr));
   const hipStream_t s = ctx_->compute_stream();
   ctx_->timer().reset();
-  if (!kPersistentLaunch || ctx_->gemm_mode() != GemmModeFP64) {
-    // Block by block on the single-product engine. FP64: independent blocks go
-    // round robin to the compute stream and the context's batch streams, so
-    // that small products run side by side, with one synchronization. Split
-    // precision: the same lanes, staged so that all blocks share one host
-    // synchronization for their plans (execute_split_batch).
-    const bool split = ctx_->gemm_mode() != GemmModeFP64;
-    constexpr int kLanes = 1 + spla::ContextInternal::kBatchStreams;
-    hipStream_t lanes[kLanes] = {s};
-    int nLanes = 1;
+  const bool alphaZero = alpha_[0] == 0.0 && alpha_[1] == 0.0;
+  const int mode = ctx_->gemm_mode();
+  if (mode == GemmModeFP64 || alphaZero) {
+    // the grouped GEMM and the split reduction, replayed from the graph
+    PhaseScope ps(ctx_->timer(), PhaseGemmF64, s);
+    full_.run(s);
+  } else {
+    // Split precision, or the automatic choice. Automatic mode: the blocks
+    // whose shape rules the split form out (a lower bound of its cost with the
+    // fewest moduli any data could need against the FP64 estimate) run as one
+    // grouped FP64 launch on the compute stream; the others, and every block in
+    // split mode, go block by block over the context's batch streams, staged so
+    // that all of them share one host synchronization for their plans
+    // (execute_split_batch; blocks whose plan falls back run the FP64 kernels).
+    std::vector<BlockSpec> fp64, split;
+    if (mode == GemmModeAuto) {
+      std::vector<char> mask(specs_.size(), 0);
+      for (std::size_t i = 0; i < specs_.size(); ++i) {
+        const BlockSpec& b = specs_[i];
+        if (b.m == 0 || b.n == 0) continue;
+        mask[i] = may_split(b) ? 1 : 0;
+        (mask[i] ? split : fp64).push_back(b);
+      }
+      if (!fp64.empty() && (!autoLaunch_ || mask != autoMask_)) {
+        autoLaunch_ = std::make_unique<GroupedLaunch>();
+        autoLaunch_->build(*ctx_, complex_, opA_, fp64, alpha_, beta_, best_schedule(fp64),
+                           /*withGraph=*/true);
+        autoMask_ = mask;
+      }
+    } else {
+      split = specs_;
+    }
+    // lanes of the block-by-block products: the batch streams, and the compute
+    // stream too when no grouped launch occupies it
+    constexpr int kMaxLanes = 1 + spla::ContextInternal::kBatchStreams;
+    hipStream_t lanes[kMaxLanes];
+    int nLanes = 0;
+    if (fp64.empty()) lanes[nLanes++] = s;
     hipEvent_t start;
     hip_check(hipEventCreateWithFlags(&start, hipEventDisableTiming));
     hip_check(hipEventRecord(start, s));
@@ -423,60 +387,48 @@ void BatchPlan::execute() {
       hip_check(hipStreamWaitEvent(lanes[nLanes++], start, 0));
     }
     (void)hipEventDestroy(start);
+    if (!fp64.empty()) {
+      PhaseScope ps(ctx_->timer(), PhaseGemmF64, s);
+      autoLaunch_->run(s);
+    }
     std::vector<Block> keep;
-    if (split) {
+    if (!split.empty()) {
       if (complex_) {
         using Z = std::complex<double>;
-        execute_split_batch<Z>(*ctx_, specs_, opA_, Z(alpha_[0], alpha_[1]),
+        execute_split_batch<Z>(*ctx_, split, opA_, Z(alpha_[0], alpha_[1]),
                                Z(beta_[0], beta_[1]), lanes, nLanes, keep);
       } else {
-        execute_split_batch<double>(*ctx_, specs_, opA_, alpha_[0], beta_[0], lanes, nLanes, keep);
+        execute_split_batch<double>(*ctx_, split, opA_, alpha_[0], beta_[0], lanes, nLanes,
+                                    keep);
       }
-    } else {
-      Block reuse[kLanes];
-      int next = 0;
-      for (const BlockSpec& b : specs_) {
-        if (b.m == 0 || b.n == 0) continue;
-        const int lane = next;
-        next = (next + 1) % nLanes;
-        const hipStream_t ls = lanes[lane];
-        FillSpec fill;
-        fill.mode = b.fillMode;
-        fill.rowOff = b.rowOffset;
-        fill.colOff = b.colOffset;
-        if (complex_) {
-          using Z = std::complex<double>;
-          enqueue_on_device<Z>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
-                               static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B),
-                               b.ldb, Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_,
-                               fill, ls, keep, &reuse[lane]);
-        } else {
-          enqueue_on_device<double>(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
-                                    static_cast<const double*>(b.A), b.lda,
-                                    static_cast<const double*>(b.B), b.ldb, beta_[0],
-                                    static_cast<double*>(b.C), b.ldc, *ctx_, fill, ls, keep,
-                                    &reuse[lane]);
-        }
-      }
-      for (Block& r : reuse) keep.push_back(std::move(r));
     }
-    for (int i = 1; i < nLanes; ++i) {  // the compute stream joins the batch streams
+    for (int i = 0; i < nLanes; ++i) {  // the compute stream joins the batch streams
+      if (lanes[i] == s) continue;
       hipEvent_t done;
       hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
       hip_check(hipEventRecord(done, lanes[i]));
       hip_check(hipStreamWaitEvent(s, done, 0));
       (void)hipEventDestroy(done);
     }
-    hip_check(hipStreamSynchronize(s));
-    if (ctx_->timer().enabled()) ctx_->timer().resolve();
-    return;
-  }
-  if (exec_) {
-    PhaseScope ps(ctx_->timer(), PhaseGemmF64, s);
-    hip_check(hipGraphLaunch(exec_, s));
   }
   hip_check(hipStreamSynchronize(s));
   if (ctx_->timer().enabled()) ctx_->timer().resolve();
+}
+
+bool BatchPlan::may_split(const BlockSpec& b) const {
+  const double fp64 = fp64_estimate_seconds(*ctx_, complex_, opA_, SPLA_OP_NONE, b.m, b.n, b.k,
+                                            /*wholeChip=*/false);
+  if (complex_) {
+    using Z = std::complex<double>;
+    SplitProduct<Z> p(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
+                      static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B), b.ldb,
+                      Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_, FillSpec());
+    return p.may_beat_fp64(fp64);
+  }
+  SplitProduct<double> p(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
+                         static_cast<const double*>(b.A), b.lda, static_cast<const double*>(b.B),
+                         b.ldb, beta_[0], static_cast<double*>(b.C), b.ldc, *ctx_, FillSpec());
+  return p.may_beat_fp64(fp64);
 }
 
 }  // namespace spla_amd

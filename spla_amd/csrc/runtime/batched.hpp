@@ -1,14 +1,20 @@
 // Batched ring-step block products: many independent
-// C_b <- alpha op(A_b) B_b + beta C_b (op = transpose or conjugate transpose,
-// the block products of one pgemm_ssb / pgemm_ssbtr ring step,
-// src/pgemm_ssb/ring_ssb_gpu.cpp:193-197) executed by one persistent launch
-// that is captured once in a hipGraph and replayed.
+// C_b <- alpha op(A_b) B_b + beta C_b, the block products of one pgemm_ssb /
+// pgemm_ssbtr ring step (op(A) = A^T or A^H, src/pgemm_ssb/ring_ssb_gpu.cpp:193-197)
+// or of one pgemm_sbs ring step (op(A) = A, src/pgemm_sbs/ring_sbs_gpu.cpp:57-88).
 //
-// Autotuning: for each tile shape the plan picks per-block split factors from
-// the target item cost and simulates the persistent schedule (items in
-// decreasing cost order on the CU slots, LPT); the cheaper shape wins. In
-// measured mode the candidate plans are also run on scratch copies of C and
-// the fastest is kept.
+// FP64 (and alpha == 0): one grouped GEMM launch over the tiles and reduction
+// splits of every block and one fixed-order reduction launch over the split
+// tiles, both nodes of one hipGraph built at planning time and replayed by
+// every execute(). Split precision and the automatic mode run block by block
+// on the single-product engine over four streams.
+//
+// Autotuning: for each tile shape the schedule (batch_schedule.hpp) picks the
+// per-block split counts from a cost model of the launch; the cheaper shape
+// wins. In measured mode six candidates (two tile shapes x three split
+// scales) also run on scratch copies of C, the fastest is kept, and its
+// measured time corrects the context's efficiency of that tile shape for the
+// batches it plans later.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,7 +23,9 @@
 #include <vector>
 
 #include "kernels/batched_desc.hpp"
+#include "kernels/launch.hpp"
 #include "runtime/allocators.hpp"
+#include "runtime/batch_schedule.hpp"
 #include "runtime/context_internal.hpp"
 #include "spla/types.h"
 
@@ -35,15 +43,32 @@ struct BlockSpec {
   int rowOffset, colOffset;
 };
 
-struct BatchCandidate {
-  int cfg = 0;
-  double splitScale = 1.0;
-  std::vector<BatchBlock> blocks;
-  std::vector<BatchItem> items;
-  int64_t workDoubles = 0;
-  int counters = 0;
-  int workgroups = 0;
-  double modelSeconds = 0.0;
+// One grouped launch (GEMM over the items, then the split reduction) over a
+// list of blocks: device copies of a schedule and, optionally, the hipGraph
+// holding both launches.
+class GroupedLaunch {
+public:
+  GroupedLaunch() = default;
+  ~GroupedLaunch();
+  GroupedLaunch(const GroupedLaunch&) = delete;
+  GroupedLaunch& operator=(const GroupedLaunch&) = delete;
+
+  void build(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
+             const std::vector<BlockSpec>& specs, const double* alpha, const double* beta,
+             const BatchSchedule& sched, bool withGraph);
+  void run(hipStream_t s);     // the graph (or the two launches)
+  void launch(hipStream_t s);  // the two launches, without the graph
+  unsigned workgroups() const { return numItems_ + numReduce_; }
+
+private:
+  void release_graph();
+
+  Block dBlocks_, dItems_, dReduce_, dWork_;
+  BatchDesc desc_{};
+  BatchKernels kernels_{};
+  unsigned numItems_ = 0, numReduce_ = 0;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
 };
 
 class BatchPlan {
@@ -56,29 +81,30 @@ public:
 
   void execute();
 
-  int tile_config() const { return cand_.cfg; }
-  int num_items() const { return static_cast<int>(cand_.items.size()); }
-  int workgroups() const { return cand_.workgroups; }
-  double model_ms() const { return cand_.modelSeconds * 1e3; }
+  int tile_config() const { return sched_.cfg; }
+  int num_items() const { return static_cast<int>(sched_.items.size()); }
+  // workgroups of the GEMM launch (one per item) plus those of the reduction
+  int workgroups() const { return static_cast<int>(full_.workgroups()); }
+  double model_ms() const { return sched_.modelSeconds * 1e3; }
   double measured_ms() const { return measuredMs_; }
 
 private:
-  BatchCandidate build(int cfg, double splitScale) const;
-  void upload(const BatchCandidate& c, std::vector<BatchBlock>* override = nullptr);
-  void launch(hipStream_t s);
-  double time_candidate(const BatchCandidate& c);
-  void capture();
+  BatchSchedule schedule(const std::vector<BlockSpec>& specs, int cfg, double splitScale) const;
+  BatchSchedule best_schedule(const std::vector<BlockSpec>& specs) const;
+  double time_candidate(const BatchSchedule& c);
+  // automatic mode: whether the split form could beat FP64 on this block's shape
+  bool may_split(const BlockSpec& b) const;
 
   std::shared_ptr<spla::ContextInternal> ctx_;
   bool complex_;
   SplaOperation opA_;
   std::vector<BlockSpec> specs_;
   double alpha_[2], beta_[2];
-  BatchCandidate cand_;
-  Block dBlocks_, dItems_, dWork_, dCounters_;
-  BatchDesc desc_{};
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t exec_ = nullptr;
+  BatchSchedule sched_;
+  GroupedLaunch full_;
+  // automatic mode: the grouped launch of the blocks that stay on FP64
+  std::unique_ptr<GroupedLaunch> autoLaunch_;
+  std::vector<char> autoMask_;
   double measuredMs_ = 0.0;
 };
 

@@ -110,6 +110,14 @@ hipStream_t ContextInternal::batch_stream(int i) {
   return batch_[i];
 }
 
+double ContextInternal::batch_efficiency(int cfg) const {
+  return batchEfficiency_[cfg == 1 ? 1 : 0];
+}
+
+void ContextInternal::set_batch_efficiency(int cfg, double e) {
+  batchEfficiency_[cfg == 1 ? 1 : 0] = e;
+}
+
 int ContextInternal::num_cus() {
   ensure_device_resources();
   return numCUs_;

@@ -68,6 +68,11 @@ public:
   int num_cus();
 
   spla_amd::Tuner& tuner() { return tuner_; }
+  // FP64 matrix rate a tile of the batched GEMM sustains (fraction of the peak)
+  // per tile configuration: the model's value until a measured batch plan
+  // corrects it.
+  double batch_efficiency(int cfg) const;
+  void set_batch_efficiency(int cfg, double e);
 
   // Arithmetic of FP64 / complex-FP64 products (include/spla_amd/spla_amd.h).
   int gemm_mode() const { return gemmMode_; }
@@ -107,6 +112,7 @@ private:
   std::size_t deviceMemory_ = 0;
 
   spla_amd::Tuner tuner_;
+  double batchEfficiency_[2] = {0.75, 0.55};  // TileL, TileM
   int gemmMode_ = 0;
   double splitTolerance_ = 1e-14;
   int splitMaxModuli_ = 20;

@@ -127,6 +127,14 @@ struct Scalar<std::complex<float>> {
 
 // One product on device-resident operands, enqueued on `stream`. Workspace
 // blocks are appended to `keep` and must live until the stream is drained.
+// Skinny kernel: products with at least this embedded reduction length, split so
+// that every split holds at least kSkinnyMinKpPerSplit (16 steps per wave),
+// kSkinnyWgPerCU workgroups per CU.
+constexpr int64_t kSkinnyMinKp = 8192;
+constexpr int64_t kSkinnyMinKpPerSplit = 1024;
+constexpr int kSkinnyWgPerCU = 4;
+static_assert(kSkinnyMinKp >= 2 * kSkinnyMinKpPerSplit, "the skinny kernel writes split partials");
+
 template <typename T>
 void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                             T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
@@ -170,7 +178,21 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
 
   const int la = (opA == SPLA_OP_NONE) ? MNMajor : KMajor;
   const int lb = (opB == SPLA_OP_NONE) ? KMajor : MNMajor;
-  const GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus(), wholeChip);
+  GemmPlan plan = ctx.tuner().plan(cplx, la, lb, m, n, k, ctx.num_cus(), wholeChip);
+  // Small m, n with a long reduction on a whole device: HBM-bound, streamed by
+  // the skinny kernel (one workgroup per split, no LDS staging).
+  // (kSkinnyMinKp / kSkinnyMinKpPerSplit >= 2: always several splits)
+  const bool skinny = !f32 && wholeChip && skinny_supported(cplx, la, lb, m, n) &&
+                      d.Kp >= kSkinnyMinKp;
+  if (skinny) {
+    const int64_t maxSplits = std::max<int64_t>(1, d.Kp / kSkinnyMinKpPerSplit);
+    int64_t sp = std::min<int64_t>(static_cast<int64_t>(ctx.num_cus()) * kSkinnyWgPerCU, maxSplits);
+    const int64_t kSplit = (d.Kp + sp - 1) / sp;
+    plan.kSplit = (kSplit + 15) / 16 * 16;
+    plan.splits = static_cast<int>((d.Kp + plan.kSplit - 1) / plan.kSplit);
+    plan.cfg = TileM;  // tile grid of the reduction's fill predicate
+    plan.tilesM = plan.tilesN = 1;
+  }
   if constexpr (!f32) {
     if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
         enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
@@ -201,6 +223,8 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
     PhaseScope ps(ctx.timer(), PhaseGemmF64, stream);
     if (f32)
       hip_check(launch_gemm_f32(d, cplx, la, lb, plan.cfg, stream));
+    else if (skinny)
+      hip_check(launch_skinny_gemm(d, cplx, stream));
     else
       hip_check(launch_gemm_f64(d, cplx, la, lb, plan.cfg, stream));
   }

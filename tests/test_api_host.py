@@ -186,8 +186,12 @@ def test_parameter_errors_precede_device_work():
 def test_batch_parameter_errors_precede_device_work():
     ctx = spla.Context(spla.ProcessingUnit.HOST)
     good = dict(m=4, n=4, k=4, A=1, lda=4, B=2, ldb=4, C=3, ldc=4)
-    with pytest.raises(spla.InvalidParameterError):  # opA must be T or C
-        spla.Batch(ctx, [good], 1.0, 0.0, opA=spla.Operation.NONE)
+    # sbs blocks (opA = N): A is m x k (lda >= m) and no fill mode
+    with pytest.raises(spla.InvalidParameterError):
+        spla.Batch(ctx, [{**good, "m": 5, "ldc": 5}], 1.0, 0.0, opA=spla.Operation.NONE)
+    with pytest.raises(spla.InvalidParameterError):
+        spla.Batch(ctx, [{**good, "fill": spla.FillMode.LOWER}], 1.0, 0.0,
+                   opA=spla.Operation.NONE)
     for bad in (dict(m=-1), dict(lda=3), dict(ldb=2), dict(ldc=3), dict(fill=7),
                 dict(row_offset=-1)):
         with pytest.raises(spla.InvalidParameterError):

@@ -2,21 +2,23 @@
 
 The batch runs many independent C_b <- alpha op(A_b) B_b + beta C_b, the block
 products of one pgemm_ssb / pgemm_ssbtr ring step
-(src/pgemm_ssb/ring_ssb_gpu.cpp:193-197). The persistent launch replayed from a
-hipGraph is switched off in the library (its first MI355X run gave no result
-within three minutes, cause not identified); the blocks then run one by one on
-the single-product engine, which these tests check. Checked here:
+(src/pgemm_ssb/ring_ssb_gpu.cpp:193-197) or of one pgemm_sbs ring step
+(op(A) = A, src/pgemm_sbs/ring_sbs_gpu.cpp:57-88). In FP64 mode a batch is one
+grouped GEMM launch over the tiles and reduction splits of every block plus one
+fixed-order reduction launch, both replayed from a hipGraph; in split precision
+the blocks run one by one on the single-product engine. Checked here:
 
-* every block against the double-double oracle (normwise error <= 1e-13),
-  real and complex, opA transpose and conjugate transpose, ragged sizes
-  including m, n or k equal to zero, ld padding and beta != 0;
+* every block against the oracle (normwise error <= 1e-13), real and complex,
+  opA transpose, conjugate transpose and (sbs) none, ragged sizes including m,
+  n or k equal to zero, ld padding and beta != 0;
 * fill modes with tile granularity (SPLA's block_is_active,
   src/block_generation/block.hpp:58-66): the requested triangle is exact, no
   element outside the block is written, and elements of the block outside the
   triangle hold either their old value or the product;
-* split-K items (large k) reduced by the last arriving workgroup: results are
-  bitwise identical over repeated graph replays, with unrelated GPU work in
-  between, and across measured / modelled tuning;
+* blocks with long reductions (split partial sums reduced in a fixed order):
+  results are bitwise identical over repeated graph replays, with unrelated GPU
+  work in between, and across measured and modelled tuning;
+* beta accumulation over replays, alpha = 0 (C <- beta C without reading A, B);
 * the split-precision path of a batch (gemm mode SPLIT_INT8).
 """
 
@@ -191,8 +193,6 @@ def test_batch_beta_accumulates_over_replays(ctx, device):
         assert rel_error(got, to_matrix(ref, m, n, ldc), A, B) <= 3e-13
 
 
-@pytest.mark.skip(reason="persistent batched launch switched off (no result within 3 minutes on "
-                         "its first MI355X run, cause not identified); measured tuning times it")
 def test_batch_measured_tuning_matches_model(ctx, device):
     sizes = [(512, 512, 8000)] * 3 + [(128, 64, 20000)] * 5
     specs, host = make_blocks(12, sizes, np.complex128, device)
@@ -228,3 +228,75 @@ def test_batch_rejects_host_pointers(ctx, device):
     specs[0]["A"] = A
     with pytest.raises(spla.GPUInvalidDevicePointerError):
         spla.Batch(ctx, specs, 1.0, 0.0)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64], ids=["z", "d"])
+def test_batch_sbs_blocks(ctx, device, dtype):
+    """sbs blocks: C_b <- alpha A_b B_b + beta C_b with A_b m x k (lda >= m)."""
+    import torch
+
+    rng = np.random.default_rng(43)
+    sizes = [(int(rng.integers(1, 400)), int(rng.integers(1, 200)), int(rng.integers(0, 3000)))
+             for _ in range(12)] + [(300, 64, 30000), (5, 7, 0)]
+    cplx = dtype == np.complex128
+    alpha = (0.25 + 1j) if cplx else 1.5
+    beta = (-0.5 + 0.5j) if cplx else 0.25
+    specs, host = [], []
+    for i, (m, n, k) in enumerate(sizes):
+        lda, ldc = m + 2, m + 1
+        A = make_buffer(44, 3 * i + 1, m, k, lda, dtype)
+        B = make_buffer(44, 3 * i + 2, k, n, max(1, k), dtype)
+        C = make_buffer(44, 3 * i + 3, m, n, ldc, dtype)
+        dA, dB, dC = (to_device(x, device) for x in (A, B, C))
+        specs.append(dict(m=m, n=n, k=k, A=dA, lda=lda, B=dB, ldb=max(1, k), C=dC, ldc=ldc))
+        host.append((A, B, C))
+    torch.cuda.synchronize()
+    batch = spla.Batch(ctx, specs, alpha, beta, complex_=cplx, opA=spla.Operation.NONE)
+    batch.execute()
+    for s, (A, B, C) in zip(specs, host):
+        m, n, k, ldc = s["m"], s["n"], s["k"], s["ldc"]
+        ref = C.copy()
+        oracle.gemm(0, 0, m, n, k, alpha, A, s["lda"], B, s["ldb"], beta, ref, ldc)
+        got = s["C"].cpu().numpy()
+        pad = got[: ldc * n].reshape(n, ldc)[:, m:]
+        assert np.array_equal(pad, C[: ldc * n].reshape(n, ldc)[:, m:])
+        assert rel_error(to_matrix(got, m, n, ldc), to_matrix(ref, m, n, ldc), A, B) <= 1e-13
+    assert batch.info()["items"] > len(sizes)
+    batch.close()
+
+
+def test_batch_alpha_zero_scales_c_without_reading_operands(ctx, device):
+    import torch
+
+    sizes = [(70, 50, 300), (33, 200, 5000)]
+    specs, host = make_blocks(15, sizes, np.complex128, device)
+    for s in specs:
+        s["A"].fill_(complex(float("nan"), 0.0))
+        s["B"].fill_(complex(float("nan"), 0.0))
+    torch.cuda.synchronize()
+    batch = spla.Batch(ctx, specs, 0.0, 0.5 - 0.5j)
+    batch.execute()
+    for s, (_, _, C) in zip(specs, host):
+        got = s["C"].cpu().numpy()
+        assert np.array_equal(got, (0.5 - 0.5j) * C)
+    batch.close()
+
+
+def test_batch_automatic_mode_mixes_grouped_fp64_and_split(device):
+    """Automatic mode: blocks whose shape rules the split form out run as one grouped
+    FP64 launch, the others block by block (split precision where it is expected to be
+    faster); every block meets the FP64 bar, and two executions agree bitwise."""
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_gemm_mode(spla.GemmMode.AUTO)
+    sizes = [(40, 30, 5000), (64, 100, 20000), (200, 48, 9000), (1024, 1024, 40000),
+             (96, 96, 3000), (3, 2, 100)]
+    specs, host = make_blocks(16, sizes, np.complex128, device)
+    batch = spla.Batch(ctx, specs, 1.0 - 0.5j, 0.0)
+    batch.execute()
+    first = [s["C"].cpu().numpy() for s in specs]
+    for s, h in zip(specs, host):
+        check_block(s, h, 2, 1.0 - 0.5j, 0.0)
+    batch.execute()
+    for s, f in zip(specs, first):
+        assert np.array_equal(s["C"].cpu().numpy(), f)
+    batch.close()

@@ -56,3 +56,43 @@ def test_accurate_dot_exact_on_cancelling_sums():
     exact = sum(Fraction(a) * Fraction(b) for a, b in zip(x, y))
     got = bench._accurate_dot(x[None, :], y[None, :])[0]
     assert got == float(exact) == 1.5
+
+
+def test_accurate_products_grid_matches_double_double_oracle():
+    # the bench's accuracy gate runs accurate_products on the GPU; the same torch
+    # code on CPU tensors against the oracle (complex, conjugated rows of A^H)
+    import torch
+
+    rng = np.random.default_rng(11)
+    k, R, C = 2500, 5, 4
+    A = (rng.uniform(-1, 1, (R, k)) + 1j * rng.uniform(-1, 1, (R, k))) * np.exp2(
+        rng.integers(-20, 20, (R, k)))
+    B = rng.uniform(-1, 1, (C, k)) + 1j * rng.uniform(-1, 1, (C, k))
+    got = bench.accurate_products(torch.from_numpy(A).conj().resolve_conj(),
+                                  torch.from_numpy(B), rows_per_pass=2).numpy()
+    for r in range(R):
+        for c in range(C):
+            ref = np.zeros(1, dtype=np.complex128)
+            oracle.gemm(2, 0, 1, 1, k, 1.0, np.ascontiguousarray(A[r]), k,
+                        np.ascontiguousarray(B[c]), k, 0.0, ref, 1, accurate=True)
+            assert abs(got[r, c].real - ref[0].real) <= 2 * np.spacing(abs(ref[0].real))
+            assert abs(got[r, c].imag - ref[0].imag) <= 2 * np.spacing(abs(ref[0].imag))
+
+
+def test_accurate_products_exact_on_cancelling_sums():
+    import torch
+
+    rng = np.random.default_rng(9)
+    k = 300
+    x = rng.integers(-2**20, 2**20, k).astype(np.float64) * 2.0**30
+    y = rng.integers(-2**20, 2**20, k).astype(np.float64)
+    X = torch.from_numpy(np.concatenate([x, -x, [3.0]])[None, :])
+    Y = torch.from_numpy(np.concatenate([y, y, [0.5]])[None, :])
+    assert bench.accurate_products(X, Y)[0, 0].item() == 1.5
+
+
+def test_sampled_grid_rows_and_columns_are_distinct():
+    rows, cols = bench.sampled_rows_cols(2048, 2048, 64, 3)
+    assert len(set(rows)) == 64 == len(set(cols))
+    rows, cols = bench.sampled_rows_cols(13, 2048, 64, 3)
+    assert sorted(rows) == list(range(13))

@@ -51,7 +51,10 @@ def case(ctx, device, dtype, opA, opB, m, n, k, kind, where="device", alpha=2.0,
 
 
 COMBOS = [("N", "N"), ("C", "N"), ("T", "C"), ("N", "T"), ("C", "C")]
-SHAPES = [(13, 32, 263), (263, 5, 40), (300, 200, 150), (1024, 768, 96), (40, 24, 20000)]
+# (13, 32, 8 / 16 / 24): the 64 x 64 tile (loads two k-blocks ahead) with one and two
+# k-blocks in a split, real and complex
+SHAPES = [(13, 32, 263), (263, 5, 40), (300, 200, 150), (1024, 768, 96), (40, 24, 20000),
+          (13, 32, 8), (13, 32, 16), (13, 32, 24)]
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64], ids=["s", "c"])

@@ -146,6 +146,35 @@ def test_small_mn_long_reduction(ctx, device, dtype, shape):
     assert np.array_equal(outs[0], outs[1])
 
 
+# the streaming kernel for small m, n (K-major A and B, long reductions): every
+# fragment-count variant (complex rows per fragment 8, real 16; columns 16), partial
+# last steps (k not a multiple of the 8-element sub-block), rows / columns past m / n
+SKINNY = {np.complex128: [(8, 16, 100003), (5, 30, 50001), (16, 9, 77777), (20, 17, 40000),
+                          (32, 32, 60000), (1, 1, 9000)],
+          np.float64: [(16, 16, 100001), (3, 31, 50000), (30, 12, 77779), (64, 32, 30001),
+                       (40, 3, 20000)]}
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64], ids=["z", "d"])
+@pytest.mark.parametrize("op", ["C", "T"])
+def test_skinny_streaming_shapes(ctx, device, dtype, op):
+    if dtype == np.float64 and op == "C":
+        pytest.skip("conjugation is the identity for real operands")
+    for m, n, k in SKINNY[dtype]:
+        A, B, C0, got, ref, sa, sb, ldc = run_case(ctx, device, dtype, op, "N", m, n, k,
+                                                   alpha=(0.5 - 2j) if dtype == np.complex128
+                                                   else 1.5, beta=0.25)
+        err = rel_error(to_matrix(got, m, n, ldc), to_matrix(ref, m, n, ldc), to_matrix(A, *sa),
+                        to_matrix(B, *sb))
+        assert err <= TOL, (m, n, k, err)
+        pad = got[: ldc * n].reshape(n, ldc)[:, m:]
+        assert np.all(pad == 1234.5), (m, n, k)
+        # integer-valued inputs: every partial sum exact, so the result is bit-exact
+        _, _, _, gi, ri, _, _, _ = run_case(ctx, device, dtype, op, "N", m, n, min(k, 40000),
+                                            kind="int", alpha=2.0, beta=-1.0)
+        assert np.array_equal(gi, ri), (m, n, k)
+
+
 def test_run_to_run_bitwise_deterministic(ctx, device):
     m, n, k = 96, 80, 30000  # split-K active
     A = make_buffer(9, 1, k, m, k, np.complex128)

@@ -203,3 +203,53 @@ def test_error_behaviour(ctx, device):
     # sbs: negative offsets (pgemm_sbs_gpu.cpp:242-244)
     with pytest.raises(spla.InvalidParameterError):
         spla.pgemm_sbs(2, 2, 2, 1.0, buf, 2, buf, 2, -1, 0, dist, 0.0, buf, 2, ctx)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32, np.complex64],
+                         ids=["d", "z", "s", "c"])
+@pytest.mark.parametrize("form", ["fp64", "split"])
+@pytest.mark.parametrize("fill", [spla.FillMode.UPPER, spla.FillMode.LOWER])
+def test_host_c_fill_beta_zero(device, dtype, form, fill):
+    # Host-resident C, beta = 0 and a fill mode: C is staged to the device, the
+    # tiles outside the triangle are skipped, and the whole staged C is copied
+    # back. Entries outside the triangle must hold the caller's value or the
+    # product (pgemm_ssbtr.h: "may or may not be computed"), never stale device
+    # memory; entries outside the sub-matrix stay bit-exact.
+    single = np.dtype(dtype) in (np.dtype(np.float32), np.dtype(np.complex64))
+    if single and form == "split":
+        pytest.skip("the split form is FP64-only")
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_gemm_mode(spla.GemmMode.SPLIT_INT8 if form == "split" else spla.GemmMode.FP64)
+    m, n, k, ro, co = 520, 610, 700, 9, 140
+    gm, gn = m + ro + 4, n + co + 3
+    ldc = gm + 2
+    A = make_buffer(31, 1, k, m, k, dtype)
+    B = make_buffer(31, 2, k, n, k, dtype)
+    C0 = make_buffer(31, 3, gm, gn, ldc, dtype)
+    # leave stale data in the device pool first (a product of the same size)
+    junk = to_device(np.full(m * n, np.nan, dtype=dtype), device)
+    spla.gemm(2 if np.iscomplexobj(A) else 1, 0, m, n, k, 1.0, to_device(A, device), k,
+              to_device(B, device), k, 0.0, junk, m, ctx)
+    del junk
+    got = C0.copy()
+    opi = 2 if np.iscomplexobj(A) else 1
+    dist = spla.MatrixDistribution.create_blacs_block_cyclic(spla.MPI_COMM_SELF, "R", 1, 1, 64, 64)
+    spla.pgemm_ssbtr(m, n, k, opi, 1.0, A, k, B, k, 0.0, got, ldc, ro, co, fill, dist, ctx)
+    wide = np.complex128 if np.iscomplexobj(A) else np.float64
+    ref = C0.astype(wide)
+    oracle.gemm(opi, 0, m, n, k, 1.0, A.astype(wide), k, B.astype(wide), k, 0.0,
+                ref[ro + co * ldc:], ldc, accurate=True)
+    G = to_matrix(got, gm, gn, ldc).astype(wide)
+    R = to_matrix(ref, gm, gn, ldc)
+    O = to_matrix(C0, gm, gn, ldc).astype(wide)
+    rows = np.arange(gm)[:, None]
+    cols = np.arange(gn)[None, :]
+    inside = (rows >= ro) & (rows < ro + m) & (cols >= co) & (cols < co + n)
+    req = inside & ((cols >= rows) if fill == spla.FillMode.UPPER else (rows >= cols))
+    tol = 1e-6 if single else TOL
+    assert rel_error(G[req], R[req], A.astype(wide), B.astype(wide)) <= tol
+    assert np.array_equal(G[~inside], O[~inside])
+    other = inside & ~req
+    assert np.isfinite(G[other]).all()
+    close = np.abs(G[other] - R[other]) <= (1e-4 if single else 1e-10) * (1 + np.abs(R[other]))
+    assert np.all(close | (G[other] == O[other]))

@@ -125,3 +125,102 @@ def test_ring_emulation_matches_global_product(ctx, device, fill):
         ri, cj = gi[indxg2p(gi, rb, gr) == pr], gj[indxg2p(gj, cb, gc) == pc]
         outside = (ri < ro)[:, None] | (cj < co)[None, :]
         assert np.array_equal(loc[p][outside], loc0[p][outside])
+
+
+def test_sbs_ring_emulation_matches_global_product(ctx, device):
+    """A multi-rank pgemm_sbs ring on one GPU (src/pgemm_sbs/pgemm_sbs_gpu.cpp:163-230,
+    ring_sbs_gpu.cpp:114-368): P virtual ranks each hold a row stripe of A and C;
+    B (k x n at offsets bro, bco of a block-cyclic global B) travels around the ring
+    in blocks. The block shape and order are SPLA's (block_size_selection on k x n
+    with full fill, grid-wise order), which spla_amd_ssb_ring_blocks computes from
+    the same rules. Each block is assembled from the owners' local B by its pieces
+    (spla_amd_region_pieces, as RingSBSGPU::prepare does); in a tile of P blocks
+    rank p starts at block (rankOffset + p) % P and moves one block per step
+    (ring), or all ranks take block s at step s (broadcast, when a tile holds
+    fewer than 0.65 P blocks). Rank p's product of a step is
+    C_p(:, cols) <- alpha A_p(:, rows) B_blk + beta' C_p(:, cols) with beta' = beta
+    on the first visit of the column block on that rank and 1 afterwards
+    (ring_sbs_gpu.cpp:278-282); the products of one step form two sbs batches (beta
+    and 1). Every rank's C is compared with alpha A_p B + beta C_p0."""
+    import torch
+
+    gr, gc = 2, 2
+    P = gr * gc
+    rbB, cbB = 40, 48                  # block-cyclic distribution of the global B
+    k, n = 900, 520                    # B sub-matrix used by the product
+    bro, bco = 5, 7                    # its offset in the global B
+    tile = 160                         # context host tile size (ring block target)
+    mloc = [300, 0, 257, 130]          # rows of A and C on each rank
+    alpha, beta = 0.5 + 0.75j, -0.25 + 0.5j
+
+    rib, cib, blocks = spla.ssb_ring_blocks(0, gr, gc, P, k, n, bro, bco,
+                                            int(spla.FillMode.FULL), tile)
+    assert len(blocks) > P and rib < k  # several tiles, several visits of each column block
+
+    gk, gn = bro + k, bco + n
+    Bg = make_buffer(22, 2, gk, gn, gk, np.complex128).reshape(gn, gk).T.copy()
+    gi, gj = np.arange(gk), np.arange(gn)
+    locB = {}
+    for p in range(P):  # each rank's local part of the block-cyclic B
+        pr, pc = p % gr, p // gr
+        ri, cj = gi[indxg2p(gi, rbB, gr) == pr], gj[indxg2p(gj, cbB, gc) == pc]
+        L = np.zeros((local_extent(gk, rbB, pr, gr), local_extent(gn, cbB, pc, gc)),
+                     dtype=np.complex128)
+        L[np.ix_(indxg2l(ri, rbB, gr), indxg2l(cj, cbB, gc))] = Bg[np.ix_(ri, cj)]
+        locB[p] = L
+
+    lda = [max(1, m) + 3 for m in mloc]
+    ldc = [max(1, m) + 2 for m in mloc]
+    A = [make_buffer(22, 10 + p, mloc[p], k, lda[p], np.complex128) for p in range(P)]
+    C0 = [make_buffer(22, 20 + p, mloc[p], n, ldc[p], np.complex128) for p in range(P)]
+    dA = [torch.from_numpy(a).to(device) for a in A]
+    dC = [torch.from_numpy(c.copy()).to(device) for c in C0]
+
+    def assemble(r, c, nr, nc):
+        blk = np.zeros((nr, nc), dtype=np.complex128)
+        for q in spla.region_pieces(0, rbB, cbB, gr, gc, nr, nc, bro + r, bco + c):
+            g_r, g_c, s_r, s_c, l_r, l_c, q_r, q_c, rank = q
+            blk[s_r:s_r + q_r, s_c:s_c + q_c] = locB[rank][l_r:l_r + q_r, l_c:l_c + q_c]
+        return torch.from_numpy(np.asfortranarray(blk).ravel(order="F")).to(device)
+
+    visited = [set() for _ in range(P)]
+    for t0 in range(0, len(blocks), P):
+        tile_blocks = blocks[t0:t0 + P]
+        r0, c0 = tile_blocks[0][0] + bro, tile_blocks[0][1] + bco
+        rank_offset = indxg2p(r0, rbB, gr) + indxg2p(c0, cbB, gc) * gr
+        ring = len(tile_blocks) >= P * 0.65
+        dBlk = [assemble(*b) for b in tile_blocks]
+        for step in range(P):
+            specs = {True: [], False: []}
+            for p in range(P):
+                idx = (rank_offset + p + step) % P if ring else step
+                if idx >= len(tile_blocks) or mloc[p] == 0:
+                    continue
+                r, c, nr, nc = tile_blocks[idx]
+                first = c not in visited[p]
+                visited[p].add(c)
+                specs[first].append(dict(m=mloc[p], n=nc, k=nr,
+                                         A=dA[p].data_ptr() + 16 * r * lda[p], lda=lda[p],
+                                         B=dBlk[idx], ldb=nr,
+                                         C=dC[p].data_ptr() + 16 * c * ldc[p], ldc=ldc[p]))
+            for first, sp in specs.items():
+                if sp:
+                    batch = spla.Batch(ctx, sp, alpha, beta if first else 1.0,
+                                       opA=spla.Operation.NONE)
+                    batch.execute()
+                    batch.close()
+
+    Bsub = np.asfortranarray(Bg[bro:, bco:]).ravel(order="F")
+    for p in range(P):
+        got = dC[p].cpu().numpy()
+        if mloc[p] == 0:
+            assert np.array_equal(got, C0[p])
+            continue
+        assert all(c in visited[p] for _, c, _, _ in blocks)
+        ref = C0[p].copy()
+        oracle.gemm(0, 0, mloc[p], n, k, alpha, A[p], lda[p], Bsub, k, beta, ref, ldc[p],
+                    accurate=True)
+        assert rel_error(got, ref, A[p], Bsub) <= 1e-13, p
+        # the ld padding of C_p is never written
+        pad = got.reshape(n, ldc[p])[:, mloc[p]:]
+        assert np.array_equal(pad, C0[p].reshape(n, ldc[p])[:, mloc[p]:])

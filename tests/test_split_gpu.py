@@ -257,3 +257,54 @@ def test_phase_timing_reports_split_phases(sctx, device):
     assert t["split_int8_gemm"] > 0 and t["split_residues"] > 0 and t["split_crt"] > 0
     assert t["gemm_fp64"] == 0
     sctx.set_timing(False)
+
+
+def test_split_workspace_limit_chunks_the_reduction(device):
+    # A workspace limit below the product's planes: the reduction runs in chunks,
+    # each a split product of its own added into C in FP64 (beta on the first).
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_gemm_mode(spla.GemmMode.SPLIT_INT8)
+    m, n, k = 128, 120, 20000
+    A = make_buffer(17, 1, k, m, k, np.complex128)
+    B = make_buffer(17, 2, k, n, k, np.complex128)
+    C0 = make_buffer(17, 3, m, n, m, np.complex128)
+    ref = C0.copy()
+    oracle.gemm(2, 0, m, n, k, 0.5 + 1j, A, k, B, k, 0.25, ref, m, accurate=True)
+    dA, dB = to_device(A, device), to_device(B, device)
+    outs = []
+    for limit in (0, 256 << 20, 16 << 20):
+        ctx.set_split_workspace_limit(limit)
+        dC = to_device(C0, device)
+        spla.gemm(2, 0, m, n, k, 0.5 + 1j, dA, k, dB, k, 0.25, dC, m, ctx)
+        info = ctx.last_product_info()
+        got = dC.cpu().numpy()
+        assert rel_error(got, ref, A, B) <= 1e-13, (limit, info)
+        outs.append((got, info))
+    (whole, iw), (chunked, ic), (fp64, i8) = outs
+    assert iw["split"] and ic["split"]
+    # two chunks of 10240 reduction indices: the rounding differs from the whole product
+    assert not np.array_equal(whole, chunked)
+    # 16 MiB holds fewer than 4096 reduction indices: FP64 kernels, reason 8
+    assert not i8["split"] and i8["reason"] == 8
+
+
+def test_split_streamed_chunks_share_one_workspace(device):
+    # Host-resident operands in split precision: every streamed k-chunk is a split
+    # product of its own; they reuse one device workspace, so the device memory
+    # the context holds does not grow with the number of chunks.
+    held = []
+    for k in (8000, 32000):
+        ctx = spla.Context(spla.ProcessingUnit.GPU)
+        ctx.set_gemm_mode(spla.GemmMode.SPLIT_INT8)
+        ctx.set_stream_chunk_bytes(4 << 20)
+        m, n = 160, 96
+        A = make_buffer(18, 1, k, m, k, np.complex128)
+        B = make_buffer(18, 2, k, n, k, np.complex128)
+        C = make_buffer(18, 3, m, n, m, np.complex128)
+        ref = C.copy()
+        spla.gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.5, C, m, ctx)
+        assert ctx.last_product_info()["split"]
+        oracle.gemm(2, 0, m, n, k, 1.0, A, k, B, k, 0.5, ref, m, accurate=True)
+        assert rel_error(C, ref, A, B) <= 1e-13
+        held.append(ctx.allocated_memory_gpu())
+    assert held[1] <= 1.25 * held[0], held

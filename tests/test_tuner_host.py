@@ -100,3 +100,16 @@ def test_gemm_grid_reaches_every_kblock_count_of_small_tiles(tuner, cplx):
                         counts.add(min(p["kSplit"], left) // 16)
     assert {1, 2} <= counts
     assert any(c >= 3 and c % 2 for c in counts) and any(c >= 4 and c % 2 == 0 for c in counts)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_single_precision_short_reductions_reach_small_tile_prologue(tuner, cplx):
+    """The short-reduction shapes of the single-precision bit-exact test (m = 13,
+    n = 32, k = 8, 16, 24) run the 64 x 64 tile with splits of one and of two k-blocks."""
+    counts = set()
+    for k in (8, 16, 24):
+        p = plan(tuner, cplx, 13, 32, k)
+        assert p["cfg"] == 1, (k, p)  # TileM
+        kp = -(-(2 if cplx else 1) * k // 16) * 16
+        counts.add(min(p["kSplit"], kp) // 16)
+    assert {1, 2} <= counts
