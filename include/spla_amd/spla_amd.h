@@ -152,6 +152,26 @@ SPLA_EXPORT SplaError spla_amd_batch_info(SplaAmdBatch batch, int* tileConfig, i
 SPLA_EXPORT SplaError spla_amd_batch_destroy(SplaAmdBatch* batch);
 
 /* ------------------------------------------------------------------------
+ * Allocation-free product.
+ *
+ * No counterpart in SPLA: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). spla_zgemm on device-resident A, B and C (the same checks and
+ * errors, synchronous), always on the FP64 kernels with the plan spla_zgemm
+ * makes in FP64 mode, so both give bit-identical results. The split-K partial
+ * sums live in a 64 MiB device array of the library that each GPU holds until
+ * the process ends; no device memory is allocated (the first call on a device
+ * only looks the array up). Limits: SPLA_GPU_ALLOCATION_ERROR when the plan's
+ * partial sums (splits x n x 2m doubles) exceed 64 MiB;
+ * SPLA_GPU_INVALID_DEVICE_POINTER_ERROR for host-resident operands. Calls
+ * that overlap on one GPU share the array and race.
+ */
+SPLA_EXPORT SplaError spla_amd_zgemm_noalloc(SplaOperation opA, SplaOperation opB, int m, int n,
+                                             int k, const void* alpha, const void* A, int lda,
+                                             const void* B, int ldb, const void* beta, void* C,
+                                             int ldc, SplaContext ctx);
+
+/* ------------------------------------------------------------------------
  * Ring-step planning (host only, no device work).
  *
  * The blocks a pgemm_ssb / pgemm_ssbtr ring computes and where each piece of

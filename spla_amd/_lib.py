@@ -118,6 +118,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("spla_amd_batch_create_sbs", P(c_void_p), c_void_p, c_int, c_int, c_void_p, c_void_p,
        c_void_p, c_int)
     fn("spla_amd_batch_execute", c_void_p)
+    fn("spla_amd_zgemm_noalloc", c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+       c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p)
     fn("spla_amd_batch_info", c_void_p, P(c_int), P(c_int), P(c_int), P(c_double), P(c_double))
     fn("spla_amd_batch_destroy", P(c_void_p))
     fn("spla_amd_ssb_ring_blocks", c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,

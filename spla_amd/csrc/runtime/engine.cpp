@@ -135,12 +135,20 @@ constexpr int64_t kSkinnyMinKpPerSplit = 1024;
 constexpr int kSkinnyWgPerCU = 4;
 static_assert(kSkinnyMinKp >= 2 * kSkinnyMinKpPerSplit, "the skinny kernel writes split partials");
 
+// `fixed` (optional): caller-provided split-K workspace; nothing is allocated,
+// and a plan whose partial sums exceed it raises GPUAllocationError.
+struct FixedWorkspace {
+  void* ptr;
+  std::size_t bytes;
+};
+
 template <typename T>
 void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,
                             T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                             T* C, int64_t ldc, spla::ContextInternal& ctx, const FillSpec& fill,
                             hipStream_t stream, std::vector<Block>& keep, Block* reuse = nullptr,
-                            bool trySplit = true, bool wholeChip = true) {
+                            bool trySplit = true, bool wholeChip = true,
+                            const FixedWorkspace* fixed = nullptr) {
   using S = Scalar<T>;
   constexpr bool cplx = S::kComplex;
   constexpr bool f32 = S::kF32;
@@ -194,7 +202,7 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
     plan.tilesM = plan.tilesN = 1;
   }
   if constexpr (!f32) {
-    if (trySplit && ctx.gemm_mode() != GemmModeFP64 &&
+    if (trySplit && !fixed && ctx.gemm_mode() != GemmModeFP64 &&
         enqueue_split_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
                                  fill, plan.estSeconds, stream, keep, reuse))
       return;
@@ -207,7 +215,10 @@ void enqueue_device_product(SplaOperation opA, SplaOperation opB, int64_t m, int
     const std::size_t bytes =
         static_cast<std::size_t>(plan.splits) * static_cast<std::size_t>(n) * d.Mp *
         sizeof(typename S::Real);
-    if (reuse) {
+    if (fixed) {
+      if (bytes > fixed->bytes) throw spla::GPUAllocationError();
+      d.work = fixed->ptr;
+    } else if (reuse) {
       // Products enqueued on one stream may share one workspace.
       if (reuse->bytes() < bytes) {
         keep.emplace_back(std::move(*reuse));
@@ -490,6 +501,41 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
     gemm_gpu_f64<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx, fill);
   }
 }
+
+template <typename T>
+void gemm_gpu_fixed_workspace(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n,
+                              int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+                              T beta, T* C, int64_t ldc, spla::ContextInternal& ctx,
+                              void* workspace, std::size_t workspaceBytes) {
+  if (m == 0 || n == 0) return;
+  check_gemm_param(opA, opB, m, n, k, A, lda, B, ldb, C, ldc);
+  if (opA != SPLA_OP_NONE && opA != SPLA_OP_TRANSPOSE && opA != SPLA_OP_CONJ_TRANSPOSE)
+    throw spla::InvalidParameterError();
+  if (opB != SPLA_OP_NONE && opB != SPLA_OP_TRANSPOSE && opB != SPLA_OP_CONJ_TRANSPOSE)
+    throw spla::InvalidParameterError();
+  if (m > (int64_t(1) << 30) || n > (int64_t(1) << 30)) throw spla::InvalidParameterError();
+  // host memory would need device staging buffers
+  if ((k > 0 && (classify(A) != MemKind::Device || classify(B) != MemKind::Device)) ||
+      classify(C) != MemKind::Device)
+    throw spla::GPUInvalidDevicePointerError();
+  DeviceGuard guard(ctx.device());
+  hip_check(hipStreamSynchronize(nullptr));
+  ctx.timer().reset();
+  const hipStream_t stream = ctx.compute_stream();
+  std::vector<Block> keep;  // stays empty: the workspace is the caller's
+  const FixedWorkspace fw{workspace, workspaceBytes};
+  enqueue_device_product<T>(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ctx,
+                            FillSpec(), stream, keep, nullptr, /*trySplit=*/false,
+                            /*wholeChip=*/true, &fw);
+  hip_check(hipStreamSynchronize(stream));
+  if (ctx.timer().enabled()) ctx.timer().resolve();
+}
+
+template void gemm_gpu_fixed_workspace<std::complex<double>>(
+    SplaOperation, SplaOperation, int64_t, int64_t, int64_t, std::complex<double>,
+    const std::complex<double>*, int64_t, const std::complex<double>*, int64_t,
+    std::complex<double>, std::complex<double>*, int64_t, spla::ContextInternal&, void*,
+    std::size_t);
 
 template <typename T>
 void enqueue_on_device(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_t k,

@@ -40,6 +40,18 @@ void gemm_gpu(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n, int64_
               const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
               spla::ContextInternal& ctx, FillSpec fill = FillSpec());
 
+// One product on device-resident operands on the FP64 kernels with the plan
+// gemm_gpu makes for it in FP64 mode, its split-K partial sums in the caller's
+// `workspace` (256-byte aligned): no device memory is allocated. Throws
+// GPUInvalidDevicePointerError for host operands and GPUAllocationError when
+// the plan's partial sums need more than workspaceBytes. Synchronous like
+// gemm_gpu (T: std::complex<double>).
+template <typename T>
+void gemm_gpu_fixed_workspace(SplaOperation opA, SplaOperation opB, int64_t m, int64_t n,
+                              int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+                              T beta, T* C, int64_t ldc, spla::ContextInternal& ctx,
+                              void* workspace, std::size_t workspaceBytes);
+
 // One product on device-resident operands, enqueued on `stream` with no
 // synchronization (T: double or std::complex<double>). Workspace blocks are
 // appended to `keep` and must live until the stream has drained; `reuse`
