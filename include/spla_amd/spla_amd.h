@@ -87,13 +87,15 @@ SPLA_EXPORT SplaError spla_amd_ctx_last_product_info(SplaContext ctx, int* used,
  * products C_b <- alpha op(A_b) B_b + beta C_b with op(A) = A^T or A^H
  * (SPLA src/pgemm_ssb/ring_ssb_gpu.cpp:193-197 issues one vendor GEMM per
  * block); one of pgemm_sbs forms C(:, cols) <- alpha A(:, rows) B_blk +
- * beta' C(:, cols) (src/pgemm_sbs/ring_sbs_gpu.cpp:57-88, 278-282). A batch
- * is planned for one persistent launch (tile shape and per-block split-K
- * factors from a cost model of the persistent schedule); in this build that
- * launch is switched off and the blocks run one after the other on the
- * single-product engine, with the same results. All pointers must be device
- * memory. A block's fill mode and offsets follow pgemm_ssbtr: tiles wholly
- * outside the requested triangle of the global C are skipped.
+ * beta' C(:, cols) (src/pgemm_sbs/ring_sbs_gpu.cpp:57-88, 278-282). In FP64
+ * mode a batch is one GEMM launch over the tiles and reduction splits of every
+ * block plus one fixed-order reduction launch, the two nodes of one hipGraph
+ * built at creation and replayed by every execute; tile shape and per-block
+ * split counts come from a cost model of the launch or from timed candidates.
+ * Split precision runs the blocks one by one; the automatic mode runs the
+ * blocks the split form cannot win as one FP64 launch. All pointers must be
+ * device memory. A block's fill mode and offsets follow pgemm_ssbtr: tiles
+ * wholly outside the requested triangle of the global C are skipped.
  */
 typedef void* SplaAmdBatch;
 
@@ -113,8 +115,11 @@ typedef struct SplaAmdBlock {
 enum SplaAmdElement { SPLA_AMD_DOUBLE = 0, SPLA_AMD_COMPLEX_DOUBLE = 1 };
 
 enum SplaAmdTuning {
-  SPLA_AMD_TUNE_MODEL = 0,   /* cost model of the persistent schedule */
-  SPLA_AMD_TUNE_MEASURE = 1  /* also time candidate plans on scratch copies of C */
+  SPLA_AMD_TUNE_MODEL = 0,     /* cost model of the schedule; the context's dispatch */
+  SPLA_AMD_TUNE_MEASURE = 1,   /* also time candidate plans (tile shape, split scale,
+                                  dispatch) on scratch copies of C */
+  SPLA_AMD_TUNE_PERSISTENT = 2 /* cost model, persistent dispatch: one workgroup per
+                                  resident slot with a static list of tiles */
 };
 
 #ifndef __cplusplus
@@ -148,6 +153,11 @@ SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch);
  * not measured) milliseconds. Output pointers may be NULL. */
 SPLA_EXPORT SplaError spla_amd_batch_info(SplaAmdBatch batch, int* tileConfig, int* items,
                                           int* workgroups, double* modelMs, double* measuredMs);
+
+/* Dispatch of the batch's GEMM launch: 1 persistent (one workgroup per resident
+ * slot working through a static list of tiles), 0 grouped (one workgroup per
+ * tile and split). */
+SPLA_EXPORT SplaError spla_amd_batch_persistent(SplaAmdBatch batch, int* persistent);
 
 SPLA_EXPORT SplaError spla_amd_batch_destroy(SplaAmdBatch* batch);
 

@@ -332,7 +332,8 @@ class Batch:
     """
 
     def __init__(self, ctx: Context, blocks, alpha, beta, complex_: bool = True,
-                 opA: Operation = Operation.CONJ_TRANSPOSE, measure: bool = False):
+                 opA: Operation = Operation.CONJ_TRANSPOSE, measure: bool = False,
+                 persistent: bool = False):
         self._lib = ctx._lib
         self._ctx = ctx  # keeps the context alive
         arr = (_lib.SplaAmdBlock * max(1, len(blocks)))()
@@ -345,14 +346,15 @@ class Batch:
         self._alpha = np.array([alpha], dtype=dt)
         self._beta = np.array([beta], dtype=dt)
         self._h = ctypes.c_void_p()
+        tuning = 1 if measure else (2 if persistent else 0)  # SplaAmdTuning
         if int(opA) == int(Operation.NONE):
             check(self._lib.spla_amd_batch_create_sbs(
                 ctypes.byref(self._h), ctx.handle, 1 if complex_ else 0, len(blocks), arr,
-                self._alpha.ctypes.data, self._beta.ctypes.data, 1 if measure else 0))
+                self._alpha.ctypes.data, self._beta.ctypes.data, tuning))
         else:
             check(self._lib.spla_amd_batch_create_ssb(
                 ctypes.byref(self._h), ctx.handle, 1 if complex_ else 0, int(opA), len(blocks),
-                arr, self._alpha.ctypes.data, self._beta.ctypes.data, 1 if measure else 0))
+                arr, self._alpha.ctypes.data, self._beta.ctypes.data, tuning))
 
     def execute(self) -> None:
         check(self._lib.spla_amd_batch_execute(self._h))
@@ -363,8 +365,11 @@ class Batch:
         check(self._lib.spla_amd_batch_info(self._h, ctypes.byref(cfg), ctypes.byref(items),
                                             ctypes.byref(wgs), ctypes.byref(model),
                                             ctypes.byref(meas)))
+        pers = ctypes.c_int()
+        check(self._lib.spla_amd_batch_persistent(self._h, ctypes.byref(pers)))
         return {"tile_config": cfg.value, "items": items.value, "workgroups": wgs.value,
-                "model_ms": model.value, "measured_ms": meas.value}
+                "model_ms": model.value, "measured_ms": meas.value,
+                "persistent": bool(pers.value)}
 
     def close(self) -> None:
         if self._h:

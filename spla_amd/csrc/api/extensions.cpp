@@ -78,7 +78,9 @@ SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext
                                                 const SplaAmdBlock* blocks, const void* alpha,
                                                 const void* beta, SplaAmdTuning tuning) {
   if (!batch || !ctx) return SPLA_INVALID_HANDLE_ERROR;
-  if (count < 0 || (type != SPLA_AMD_DOUBLE && type != SPLA_AMD_COMPLEX_DOUBLE))
+  if (count < 0 || (type != SPLA_AMD_DOUBLE && type != SPLA_AMD_COMPLEX_DOUBLE) ||
+      (tuning != SPLA_AMD_TUNE_MODEL && tuning != SPLA_AMD_TUNE_MEASURE &&
+       tuning != SPLA_AMD_TUNE_PERSISTENT))
     return SPLA_INVALID_PARAMETER_ERROR;
   if ((count > 0 && !blocks) || !alpha || !beta) return SPLA_INVALID_POINTER_ERROR;
   SPLA_C_GUARD({
@@ -93,7 +95,7 @@ SPLA_EXPORT SplaError spla_amd_batch_create_ssb(SplaAmdBatch* batch, SplaContext
     *batch = new spla_amd::BatchPlan(ctxPtr, type == SPLA_AMD_COMPLEX_DOUBLE, opA,
                                      std::move(specs), static_cast<const double*>(alpha),
                                      static_cast<const double*>(beta),
-                                     tuning == SPLA_AMD_TUNE_MEASURE);
+                                     static_cast<int>(tuning));
   })
 }
 
@@ -103,6 +105,12 @@ SPLA_EXPORT SplaError spla_amd_batch_create_sbs(SplaAmdBatch* batch, SplaContext
                                                 const void* beta, SplaAmdTuning tuning) {
   return spla_amd_batch_create_ssb(batch, ctx, type, SPLA_OP_NONE, count, blocks, alpha, beta,
                                    tuning);
+}
+
+SPLA_EXPORT SplaError spla_amd_batch_persistent(SplaAmdBatch batch, int* persistent) {
+  if (!batch) return SPLA_INVALID_HANDLE_ERROR;
+  if (!persistent) return SPLA_INVALID_POINTER_ERROR;
+  SPLA_C_GUARD(*persistent = static_cast<spla_amd::BatchPlan*>(batch)->persistent() ? 1 : 0)
 }
 
 SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch) {

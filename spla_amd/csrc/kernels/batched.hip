@@ -1,11 +1,14 @@
-// Batched FP64 / complex-FP64 block products of one ring step: one grouped
-// GEMM launch whose workgroups are the (block, tile, split) items of every
-// block, in decreasing cost order (the dispatcher hands them to the CUs as
-// these free up), then one reduction launch over the tiles whose reduction is
-// split, which adds the partial sums in split order (bitwise reproducible) and
-// applies alpha / beta. Both launches read only descriptors and operands; the
-// partial sums are written before they are read in every launch, so the pair
-// can be replayed from a hipGraph without clearing anything.
+// Batched FP64 / complex-FP64 block products of one ring step: one GEMM launch
+// over the (block, tile, split) items of every block, then one reduction launch
+// over the tiles whose reduction is split, which adds the partial sums in split
+// order (bitwise reproducible) and applies alpha / beta. The GEMM launch is
+// grouped (one workgroup per item, in decreasing cost order: the dispatcher
+// hands them to the CUs as these free up) or persistent (one workgroup per
+// resident slot, each working through the static list the host's
+// longest-processing-time schedule gave it). Both launches read only
+// descriptors and operands; the partial sums are written before they are read
+// in every launch, so the pair can be replayed from a hipGraph without clearing
+// anything.
 //
 // ssb blocks (SPLA src/pgemm_ssb/ring_ssb_gpu.cpp:193-197): C_b <- alpha op(A_b) B_b
 // + beta C_b with op(A) = A^T / A^H, A k x m (K-major). sbs blocks
@@ -45,11 +48,13 @@ __device__ __forceinline__ void update_c(const BatchDesc& d, const BatchBlock& b
   }
 }
 
+// One (block, tile, split) item; on return every thread has passed the tile
+// routine's last barrier, so the LDS buffers may be reused at once.
 template <bool CPLX, int LA, int WM, int WN>
-__global__ void __launch_bounds__(kThreads, 2) batched_gemm_kernel(BatchDesc d) {
+__device__ __forceinline__ void batched_item(
+    const BatchDesc& d, const BatchItem& it,
+    typename TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>::Smem& sm) {
   using T = TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>;
-  __shared__ typename T::Smem sm;
-  const BatchItem it = d.items[blockIdx.x];
   const BatchBlock b = d.blocks[it.block];
   const int row0 = it.tm * T::BM;  // embedded rows
   const int col0 = it.tn * T::BN;
@@ -67,6 +72,23 @@ __global__ void __launch_bounds__(kThreads, 2) batched_gemm_kernel(BatchDesc d) 
       store_work<double, CPLX>(w, b.Mp, i, j, re, im);
     });
   }
+}
+
+// Grouped launch: workgroup = item (gridDim.x = number of items).
+template <bool CPLX, int LA, int WM, int WN>
+__global__ void __launch_bounds__(kThreads, 2) batched_gemm_kernel(BatchDesc d) {
+  __shared__ typename TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>::Smem sm;
+  batched_item<CPLX, LA, WM, WN>(d, d.items[blockIdx.x], sm);
+}
+
+// Persistent launch: one workgroup per slot works through its static list of
+// items (gridDim.x = number of slots); no workgroup waits for another.
+template <bool CPLX, int LA, int WM, int WN>
+__global__ void __launch_bounds__(kThreads, 2) batched_gemm_persistent_kernel(BatchDesc d) {
+  __shared__ typename TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>::Smem sm;
+  const int end = d.slotStart[blockIdx.x + 1];
+  for (int i = d.slotStart[blockIdx.x]; i < end; ++i)
+    batched_item<CPLX, LA, WM, WN>(d, d.items[d.slotItems[i]], sm);
 }
 
 // AR x BN: element rows and columns of a tile of the GEMM launch.
@@ -94,23 +116,28 @@ __global__ void __launch_bounds__(kThreads) batched_reduce_kernel(BatchDesc d) {
 }
 
 template <bool CPLX, int LA>
-const void* gemm_kernel_for(int cfg) {
+const void* gemm_kernel_for(int cfg, bool persistent) {
+  if (persistent) {
+    if (cfg == TileM)
+      return reinterpret_cast<const void*>(&batched_gemm_persistent_kernel<CPLX, LA, 32, 32>);
+    return reinterpret_cast<const void*>(&batched_gemm_persistent_kernel<CPLX, LA, 64, 64>);
+  }
   if (cfg == TileM) return reinterpret_cast<const void*>(&batched_gemm_kernel<CPLX, LA, 32, 32>);
   return reinterpret_cast<const void*>(&batched_gemm_kernel<CPLX, LA, 64, 64>);
 }
 
 }  // namespace
 
-BatchKernels batched_kernels(bool complex, int layoutA, int cfg) {
+BatchKernels batched_kernels(bool complex, int layoutA, int cfg, bool persistent) {
   BatchKernels k;
   if (complex) {
-    k.gemm = layoutA == MNMajor ? gemm_kernel_for<true, MNMajor>(cfg)
-                                : gemm_kernel_for<true, KMajor>(cfg);
+    k.gemm = layoutA == MNMajor ? gemm_kernel_for<true, MNMajor>(cfg, persistent)
+                                : gemm_kernel_for<true, KMajor>(cfg, persistent);
     k.reduce = cfg == TileM ? reinterpret_cast<const void*>(&batched_reduce_kernel<true, 32, 64>)
                             : reinterpret_cast<const void*>(&batched_reduce_kernel<true, 64, 128>);
   } else {
-    k.gemm = layoutA == MNMajor ? gemm_kernel_for<false, MNMajor>(cfg)
-                                : gemm_kernel_for<false, KMajor>(cfg);
+    k.gemm = layoutA == MNMajor ? gemm_kernel_for<false, MNMajor>(cfg, persistent)
+                                : gemm_kernel_for<false, KMajor>(cfg, persistent);
     k.reduce = cfg == TileM ? reinterpret_cast<const void*>(&batched_reduce_kernel<false, 64, 64>)
                             : reinterpret_cast<const void*>(&batched_reduce_kernel<false, 128, 128>);
   }

@@ -1,8 +1,10 @@
 // Descriptors of the batched block-product kernels (batched.hip): many
 // independent block products C_b <- alpha op(A_b) B_b + beta C_b (the block
-// products of one pgemm_ssb / pgemm_ssbtr or pgemm_sbs ring step) in one
-// grouped GEMM launch over the tiles of every block, followed by one
-// fixed-order reduction launch over the tiles whose reduction is split.
+// products of one pgemm_ssb / pgemm_ssbtr or pgemm_sbs ring step) in one GEMM
+// launch over the tiles of every block (grouped: one workgroup per tile and
+// split; or persistent: one workgroup per CU slot with a static list of them),
+// followed by one fixed-order reduction launch over the tiles whose reduction
+// is split.
 #pragma once
 
 #include <cstdint>
@@ -34,7 +36,11 @@ struct BatchReduceTile {
 
 struct BatchDesc {
   const BatchBlock* blocks;
-  const BatchItem* items;          // gridDim.x of the GEMM launch
+  const BatchItem* items;          // grouped GEMM launch: one workgroup per item
+  // persistent GEMM launch: workgroup w works through the items
+  // slotItems[slotStart[w] .. slotStart[w + 1]) in order (a static schedule)
+  const int* slotStart;
+  const int* slotItems;
   const BatchReduceTile* reduce;   // gridDim.x of the reduction launch
   double* work;
   double alphaRe, alphaIm, betaRe, betaIm;

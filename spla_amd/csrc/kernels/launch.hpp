@@ -45,16 +45,17 @@ hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream);
 // limbs: CrtConstants::limbs of the constants d.cc points to
 hipError_t launch_crt(bool complex, const CrtDesc& d, int limbs, hipStream_t stream);
 
-// Batched block products (batched.hip): the grouped GEMM kernel (one workgroup
-// per BatchItem, kernel argument one BatchDesc) and the split reduction kernel
-// (one workgroup per BatchReduceTile) of a tile configuration (TileL or TileM)
-// and A layout (KMajor: ssb blocks, MNMajor: sbs blocks), as kernel handles for
+// Batched block products (batched.hip): the GEMM kernel (grouped: one workgroup
+// per BatchItem; persistent: one workgroup per slot of the static schedule;
+// kernel argument one BatchDesc) and the split reduction kernel (one workgroup
+// per BatchReduceTile) of a tile configuration (TileL or TileM) and A layout
+// (KMajor: ssb blocks, MNMajor: sbs blocks), as kernel handles for
 // hipLaunchKernel and hipGraphAddKernelNode; `threads` per workgroup.
 struct BatchKernels {
   const void* gemm;
   const void* reduce;
   int threads;
 };
-BatchKernels batched_kernels(bool complex, int layoutA, int cfg);
+BatchKernels batched_kernels(bool complex, int layoutA, int cfg, bool persistent);
 
 }  // namespace spla_amd

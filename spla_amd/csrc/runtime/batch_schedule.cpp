@@ -24,6 +24,11 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 int batch_blocks_per_cu(int cfg) { return cfg == TileM ? 4 : 2; }
 
+int batch_slots(int cfg, int numCUs, std::size_t items) {
+  const std::size_t slots = static_cast<std::size_t>(std::max(1, numCUs)) * batch_blocks_per_cu(cfg);
+  return static_cast<int>(std::max<std::size_t>(1, std::min(slots, items)));
+}
+
 bool batch_tile_active(int fillMode, int64_t rowOff, int64_t colOff, int64_t r, int64_t c,
                        int64_t nr, int64_t nc) {
   if (fillMode == SPLA_FILL_MODE_UPPER) return c + nc + colOff > r + rowOff;
@@ -113,16 +118,27 @@ BatchSchedule plan_batch_schedule(const std::vector<BatchShape>& shapes, bool co
   std::stable_sort(items.begin(), items.end(),
                    [](const Costed& a, const Costed& b) { return a.cost > b.cost; });
   out.items.reserve(items.size());
-  std::priority_queue<double, std::vector<double>, std::greater<double>> load;
-  for (int i = 0; i < slots; ++i) load.push(0.0);
+  // longest processing time first: each item goes to the slot that frees up
+  // first (ties: the lowest slot), which is also the persistent launch's schedule
+  const int nSlots = batch_slots(cfg, numCUs, items.size());
+  using Slot = std::pair<double, int>;  // (load, slot)
+  std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> load;
+  for (int i = 0; i < nSlots; ++i) load.push({0.0, i});
+  std::vector<std::vector<int>> perSlot(nSlots);
   double makespan = 0.0;
-  for (const Costed& it : items) {
-    out.items.push_back(it.item);
-    double l = load.top();
+  for (std::size_t i = 0; i < items.size(); ++i) {
+    out.items.push_back(items[i].item);
+    Slot sl = load.top();
     load.pop();
-    l += it.cost;
-    makespan = std::max(makespan, l);
-    load.push(l);
+    sl.first += items[i].cost;
+    makespan = std::max(makespan, sl.first);
+    perSlot[sl.second].push_back(static_cast<int>(i));
+    load.push(sl);
+  }
+  out.slotStart.assign(1, 0);
+  for (const std::vector<int>& v : perSlot) {
+    out.slotItems.insert(out.slotItems.end(), v.begin(), v.end());
+    out.slotStart.push_back(static_cast<int>(out.slotItems.size()));
   }
   out.modelSeconds = makespan + reduceSeconds + kLaunchOverhead * (out.reduce.empty() ? 1 : 2);
   return out;

@@ -31,6 +31,10 @@ struct BatchSchedule {
   double splitScale = 1.0;  // scales the split counts of the model (tuning candidates)
   std::vector<BatchBlock> blocks;  // one per shape; pointers and leading dimensions unset
   std::vector<BatchItem> items;    // GEMM workgroups in dispatch order
+  // static schedule of the persistent launch: slot w (one workgroup of the
+  // launch) takes items slotItems[slotStart[w] .. slotStart[w + 1]) in order,
+  // as the model's longest-processing-time simulation assigned them
+  std::vector<int> slotStart, slotItems;
   std::vector<BatchReduceTile> reduce;
   int64_t workDoubles = 0;
   double modelSeconds = 0.0;
@@ -38,6 +42,8 @@ struct BatchSchedule {
 
 // Resident workgroups per CU of the batched GEMM kernel of a tile configuration.
 int batch_blocks_per_cu(int cfg);
+// Workgroups of the persistent launch: the resident slots, at most one per item.
+int batch_slots(int cfg, int numCUs, std::size_t items);
 
 // efficiency: the FP64 matrix rate a tile sustains, as a fraction of the peak.
 BatchSchedule plan_batch_schedule(const std::vector<BatchShape>& shapes, bool complex,

@@ -141,7 +141,8 @@ void GroupedLaunch::release_graph() {
 
 void GroupedLaunch::build(spla::ContextInternal& ctx, bool complex, SplaOperation opA,
                           const std::vector<BlockSpec>& specs, const double* alpha,
-                          const double* beta, const BatchSchedule& sched, bool withGraph) {
+                          const double* beta, const BatchSchedule& sched, bool persistent,
+                          bool withGraph) {
   release_graph();
   const hipStream_t s = ctx.compute_stream();
   std::vector<BatchBlock> blocks = sched.blocks;
@@ -160,6 +161,9 @@ void GroupedLaunch::build(spla::ContextInternal& ctx, bool complex, SplaOperatio
   dReduce_ = Block(ctx.gpu_memory(),
                    std::max<std::size_t>(1, sched.reduce.size()) * sizeof(BatchReduceTile));
   dWork_ = Block(ctx.gpu_memory(), std::max<int64_t>(1, sched.workDoubles) * sizeof(double));
+  dSlots_ = Block(ctx.gpu_memory(),
+                  (sched.slotStart.size() + std::max<std::size_t>(1, sched.slotItems.size())) *
+                      sizeof(int));
   if (!blocks.empty())
     hip_check(hipMemcpyAsync(dBlocks_.get(), blocks.data(), blocks.size() * sizeof(BatchBlock),
                              hipMemcpyHostToDevice, s));
@@ -170,10 +174,18 @@ void GroupedLaunch::build(spla::ContextInternal& ctx, bool complex, SplaOperatio
     hip_check(hipMemcpyAsync(dReduce_.get(), sched.reduce.data(),
                              sched.reduce.size() * sizeof(BatchReduceTile), hipMemcpyHostToDevice,
                              s));
+  if (!sched.slotStart.empty())
+    hip_check(hipMemcpyAsync(dSlots_.get(), sched.slotStart.data(),
+                             sched.slotStart.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  if (!sched.slotItems.empty())
+    hip_check(hipMemcpyAsync(dSlots_.get<int>() + sched.slotStart.size(), sched.slotItems.data(),
+                             sched.slotItems.size() * sizeof(int), hipMemcpyHostToDevice, s));
   hip_check(hipStreamSynchronize(s));
   std::memset(&desc_, 0, sizeof(desc_));
   desc_.blocks = dBlocks_.get<BatchBlock>();
   desc_.items = dItems_.get<BatchItem>();
+  desc_.slotStart = dSlots_.get<int>();
+  desc_.slotItems = dSlots_.get<int>() + sched.slotStart.size();
   desc_.reduce = dReduce_.get<BatchReduceTile>();
   desc_.work = dWork_.get<double>();
   desc_.alphaRe = alpha[0];
@@ -183,9 +195,14 @@ void GroupedLaunch::build(spla::ContextInternal& ctx, bool complex, SplaOperatio
   desc_.betaZero = (desc_.betaRe == 0.0 && desc_.betaIm == 0.0) ? 1 : 0;
   desc_.conjA = (complex && opA == SPLA_OP_CONJ_TRANSPOSE) ? 1 : 0;
   desc_.conjB = 0;
-  kernels_ = batched_kernels(complex, opA == SPLA_OP_NONE ? MNMajor : KMajor, sched.cfg);
-  numItems_ = static_cast<unsigned>(sched.items.size());
+  kernels_ = batched_kernels(complex, opA == SPLA_OP_NONE ? MNMajor : KMajor, sched.cfg,
+                             persistent);
+  // GEMM workgroups: one per item (grouped) or one per slot (persistent)
+  numItems_ = sched.items.empty() ? 0u
+                                  : static_cast<unsigned>(persistent ? sched.slotStart.size() - 1
+                                                                     : sched.items.size());
   numReduce_ = static_cast<unsigned>(sched.reduce.size());
+  persistent_ = persistent;
   if (withGraph && numItems_ > 0) {
     // the two launches as the nodes of one graph, replayed by run()
     hip_check(hipGraphCreate(&graph_, 0));
@@ -231,8 +248,9 @@ void GroupedLaunch::run(hipStream_t s) {
 
 BatchPlan::BatchPlan(std::shared_ptr<spla::ContextInternal> ctx, bool complex, SplaOperation opA,
                      std::vector<BlockSpec> blocks, const double* alpha, const double* beta,
-                     bool measure)
+                     int tuning)
     : ctx_(std::move(ctx)), complex_(complex), opA_(opA), specs_(std::move(blocks)) {
+  const bool measure = tuning == 1;
   // ssb blocks: op(A) = A^T or A^H (A k x m); sbs blocks: op(A) = A (A m x k, full fill)
   if (opA != SPLA_OP_TRANSPOSE && opA != SPLA_OP_CONJ_TRANSPOSE && opA != SPLA_OP_NONE)
     throw spla::InvalidParameterError();
@@ -264,23 +282,29 @@ BatchPlan::BatchPlan(std::shared_ptr<spla::ContextInternal> ctx, bool complex, S
   // fastest is kept and its measured rate corrects the model's efficiency for
   // the batches the context plans afterwards.
   BatchSchedule best = best_schedule(specs_);
+  bool persistent = tuning == 2 || ctx_->batch_persistent();
   if (measure && !best.items.empty()) {
     double bestMs = 1e300;
     for (int cfg : {TileL, TileM})
       for (double sc : {0.5, 1.0, 2.0}) {
         BatchSchedule c = schedule(specs_, cfg, sc);
-        const double ms = time_candidate(c);
-        if (ms < bestMs) {
-          bestMs = ms;
-          best = std::move(c);
+        for (bool pers : {false, true}) {
+          const double ms = time_candidate(c, pers);
+          if (ms < bestMs) {
+            bestMs = ms;
+            best = c;
+            persistent = pers;
+          }
         }
       }
     measuredMs_ = bestMs;
     const double eff = ctx_->batch_efficiency(best.cfg) * best.modelSeconds / (bestMs * 1e-3);
     ctx_->set_batch_efficiency(best.cfg, std::min(0.95, std::max(0.05, eff)));
+    ctx_->set_batch_persistent(persistent);
   }
   sched_ = std::move(best);
-  full_.build(*ctx_, complex_, opA_, specs_, alpha_, beta_, sched_, /*withGraph=*/true);
+  full_.build(*ctx_, complex_, opA_, specs_, alpha_, beta_, sched_, persistent,
+              /*withGraph=*/true);
 }
 
 BatchPlan::~BatchPlan() = default;
@@ -303,7 +327,7 @@ BatchSchedule BatchPlan::best_schedule(const std::vector<BlockSpec>& specs) cons
   return best;
 }
 
-double BatchPlan::time_candidate(const BatchSchedule& c) {
+double BatchPlan::time_candidate(const BatchSchedule& c, bool persistent) {
   // Run the candidate on scratch copies of C (C itself is left untouched).
   const int comps = complex_ ? 2 : 1;
   int64_t total = 0;
@@ -319,7 +343,7 @@ double BatchPlan::time_candidate(const BatchSchedule& c) {
     off += static_cast<int64_t>(b.m) * b.n * comps;
   }
   GroupedLaunch trial;
-  trial.build(*ctx_, complex_, opA_, shadow, alpha_, beta_, c, /*withGraph=*/false);
+  trial.build(*ctx_, complex_, opA_, shadow, alpha_, beta_, c, persistent, /*withGraph=*/false);
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0));
   hip_check(hipEventCreate(&e1));
@@ -367,7 +391,7 @@ void BatchPlan::execute() {
       if (!fp64.empty() && (!autoLaunch_ || mask != autoMask_)) {
         autoLaunch_ = std::make_unique<GroupedLaunch>();
         autoLaunch_->build(*ctx_, complex_, opA_, fp64, alpha_, beta_, best_schedule(fp64),
-                           /*withGraph=*/true);
+                           ctx_->batch_persistent(), /*withGraph=*/true);
         autoMask_ = mask;
       }
     } else {

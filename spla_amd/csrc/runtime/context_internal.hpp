@@ -73,6 +73,9 @@ public:
   // corrects it.
   double batch_efficiency(int cfg) const;
   void set_batch_efficiency(int cfg, double e);
+  // dispatch of the batched GEMM launch: persistent (static schedule) or grouped
+  bool batch_persistent() const { return batchPersistent_; }
+  void set_batch_persistent(bool p) { batchPersistent_ = p; }
 
   // Arithmetic of FP64 / complex-FP64 products (include/spla_amd/spla_amd.h).
   int gemm_mode() const { return gemmMode_; }
@@ -113,6 +116,7 @@ private:
 
   spla_amd::Tuner tuner_;
   double batchEfficiency_[2] = {0.75, 0.55};  // TileL, TileM
+  bool batchPersistent_ = false;
   int gemmMode_ = 0;
   double splitTolerance_ = 1e-14;
   int splitMaxModuli_ = 20;

@@ -56,5 +56,15 @@ void bs_reduce(void* h, int* out) {
   std::memcpy(out, s->reduce.data(), s->reduce.size() * sizeof(spla_amd::BatchReduceTile));
 }
 
+// static schedule of the persistent launch: slots + 1 starts, then the item list
+long long bs_num_slots(void* h) {
+  return static_cast<long long>(static_cast<spla_amd::BatchSchedule*>(h)->slotStart.size()) - 1;
+}
+void bs_slots(void* h, int* start, int* items) {
+  auto* s = static_cast<spla_amd::BatchSchedule*>(h);
+  std::memcpy(start, s->slotStart.data(), s->slotStart.size() * sizeof(int));
+  std::memcpy(items, s->slotItems.data(), s->slotItems.size() * sizeof(int));
+}
+
 void bs_free(void* h) { delete static_cast<spla_amd::BatchSchedule*>(h); }
 }

@@ -13,6 +13,8 @@ any GPU run:
   active tiles of the blocks with more than one split, once each;
 * both grids fit a launch; blocks with m, n or k = 0, and alpha = 0 (no
   reduction: C <- beta C), are handled;
+* the persistent launch's static lists (one per resident workgroup slot)
+  partition the items, each in dispatch order;
 * on the 512-block ring-step workload the split counts and tile choice are
   those the cost model prefers, and the model time is finite.
 """
@@ -53,6 +55,10 @@ def lib(tmp_path_factory):
     h.bs_blocks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ll)]
     h.bs_items.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     h.bs_reduce.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    h.bs_num_slots.restype = ctypes.c_longlong
+    h.bs_num_slots.argtypes = [ctypes.c_void_p]
+    h.bs_slots.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(ctypes.c_int)]
     h.bs_free.argtypes = [ctypes.c_void_p]
     return h
 
@@ -70,8 +76,14 @@ def plan(lib, shapes, cplx, cfg, scale=1.0, alpha_zero=False, cus=256, eff=0.75)
     red = np.zeros((cnt[1], 3), dtype=np.int32)
     lib.bs_reduce(h, red.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     model = lib.bs_model_seconds(h)
+    nslots = lib.bs_num_slots(h)
+    start = np.zeros(nslots + 1, dtype=np.int32)
+    order = np.zeros(max(1, cnt[0]), dtype=np.int32)
+    lib.bs_slots(h, start.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                 order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     lib.bs_free(h)
-    return dict(blocks=blocks, items=items, reduce=red, work=cnt[2], cfg=cnt[3], model=model)
+    return dict(blocks=blocks, items=items, reduce=red, work=cnt[2], cfg=cnt[3], model=model,
+                slot_start=start, slot_items=order[:cnt[0]], cus=cus)
 
 
 def active(fill, ro, co, r, c, nr, nc):
@@ -136,6 +148,16 @@ def emulate(p, shapes, cplx, sbs, alpha_zero=False):
         edges = [q[1] for q in parts] + [parts[-1][2]]
         assert edges[0] == 0 and edges[-1] == Kp
         assert all(parts[i][2] == parts[i + 1][1] for i in range(len(parts) - 1))
+    # persistent launch: the slots' static lists partition the items, each list in
+    # dispatch order, one slot per resident workgroup at most
+    start, order = p["slot_start"], p["slot_items"]
+    per_cu = 4 if p["cfg"] == 1 else 2
+    assert len(start) - 1 == max(1, min(p["cus"] * per_cu, len(items)))
+    assert start[0] == 0 and start[-1] == len(items) and (np.diff(start) >= 0).all()
+    assert sorted(order.tolist()) == list(range(len(items)))
+    for w in range(len(start) - 1):
+        mine = order[start[w]:start[w + 1]]
+        assert (np.diff(mine) > 0).all()
     red_set = [tuple(t) for t in red]
     assert len(red_set) == len(set(red_set))
     assert set(red_set) == {t for t in expect if blocks[t[0]][4] > 1}

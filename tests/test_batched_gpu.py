@@ -4,7 +4,7 @@ The batch runs many independent C_b <- alpha op(A_b) B_b + beta C_b, the block
 products of one pgemm_ssb / pgemm_ssbtr ring step
 (src/pgemm_ssb/ring_ssb_gpu.cpp:193-197) or of one pgemm_sbs ring step
 (op(A) = A, src/pgemm_sbs/ring_sbs_gpu.cpp:57-88). In FP64 mode a batch is one
-grouped GEMM launch over the tiles and reduction splits of every block plus one
+GEMM launch over the tiles and reduction splits of every block plus one
 fixed-order reduction launch, both replayed from a hipGraph; in split precision
 the blocks run one by one on the single-product engine. Checked here:
 
@@ -102,9 +102,10 @@ def ragged_sizes(rng, count, max_m, max_n, max_k):
     return out
 
 
+@pytest.mark.parametrize("persistent", [False, True], ids=["grouped", "persistent"])
 @pytest.mark.parametrize("dtype,opA", [(np.complex128, 2), (np.complex128, 1), (np.float64, 1)],
                          ids=["zC", "zT", "dT"])
-def test_batch_ragged_blocks(ctx, device, dtype, opA):
+def test_batch_ragged_blocks(ctx, device, dtype, opA, persistent):
     rng = np.random.default_rng(41)
     sizes = ragged_sizes(rng, 30, 300, 220, 2000)
     sizes += [(0, 17, 30), (19, 0, 30), (23, 31, 0), (1, 1, 1), (256, 256, 4096)]
@@ -112,12 +113,13 @@ def test_batch_ragged_blocks(ctx, device, dtype, opA):
     alpha = (0.75 - 0.5j) if cplx else 0.75
     beta = (0.5 + 0.25j) if cplx else -0.5
     specs, host = make_blocks(7, sizes, dtype, device, pad=(3, 1, 2))
-    batch = spla.Batch(ctx, specs, alpha, beta, complex_=cplx, opA=opA)
+    batch = spla.Batch(ctx, specs, alpha, beta, complex_=cplx, opA=opA, persistent=persistent)
     batch.execute()
     for s, h in zip(specs, host):
         check_block(s, h, opA, alpha, beta)
     info = batch.info()
     assert info["items"] >= 1 and info["workgroups"] >= 1 and info["model_ms"] > 0
+    assert info["persistent"] == persistent
     batch.close()
 
 
@@ -151,12 +153,13 @@ def test_batch_fill_modes(ctx, device, fill):
         check_block(s, h, 2, 1.0 + 0.5j, 0.5)
 
 
-def test_batch_split_k_deterministic_over_replays(ctx, device):
+@pytest.mark.parametrize("persistent", [False, True], ids=["grouped", "persistent"])
+def test_batch_split_k_deterministic_over_replays(ctx, device, persistent):
     import torch
 
     sizes = [(256, 192, 60000), (64, 64, 90000), (300, 40, 5000), (96, 128, 30000)]
     specs, host = make_blocks(10, sizes, np.complex128, device)
-    batch = spla.Batch(ctx, specs, 1.0, 0.0)
+    batch = spla.Batch(ctx, specs, 1.0, 0.0, persistent=persistent)
     info = batch.info()
     first = None
     for rep in range(4):
@@ -230,8 +233,9 @@ def test_batch_rejects_host_pointers(ctx, device):
         spla.Batch(ctx, specs, 1.0, 0.0)
 
 
+@pytest.mark.parametrize("persistent", [False, True], ids=["grouped", "persistent"])
 @pytest.mark.parametrize("dtype", [np.complex128, np.float64], ids=["z", "d"])
-def test_batch_sbs_blocks(ctx, device, dtype):
+def test_batch_sbs_blocks(ctx, device, dtype, persistent):
     """sbs blocks: C_b <- alpha A_b B_b + beta C_b with A_b m x k (lda >= m)."""
     import torch
 
@@ -251,7 +255,8 @@ def test_batch_sbs_blocks(ctx, device, dtype):
         specs.append(dict(m=m, n=n, k=k, A=dA, lda=lda, B=dB, ldb=max(1, k), C=dC, ldc=ldc))
         host.append((A, B, C))
     torch.cuda.synchronize()
-    batch = spla.Batch(ctx, specs, alpha, beta, complex_=cplx, opA=spla.Operation.NONE)
+    batch = spla.Batch(ctx, specs, alpha, beta, complex_=cplx, opA=spla.Operation.NONE,
+                       persistent=persistent)
     batch.execute()
     for s, (A, B, C) in zip(specs, host):
         m, n, k, ldc = s["m"], s["n"], s["k"], s["ldc"]
