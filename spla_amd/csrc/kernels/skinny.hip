@@ -65,7 +65,9 @@ __global__ void __launch_bounds__(kWaves * 64) skinny_gemm_kernel(GemmDesc d) {
   // this wave's steps: wave, wave + 4, wave + 8, ...
   const int64_t mySteps = steps > wave ? (steps - wave + kWaves - 1) / kWaves : 0;
 
-  // row / column bases of this lane (rows past m, columns past n read zeros)
+  // row / column bases of this lane; rows past m and columns past n read row /
+  // column 0 and are zeroed after the load, so that every load is issued
+  // unconditionally and the waits for the register ring stay static
   const double* pa[FM];
   bool va[FM];
 #pragma unroll
@@ -91,39 +93,39 @@ __global__ void __launch_bounds__(kWaves * 64) skinny_gemm_kernel(GemmDesc d) {
 #pragma unroll
     for (int g = 0; g < FN; ++g) acc[f][g] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-  // loads of wave step t (embedded indices q0 .. q0 + 15) into ring slot u:
-  // sub-block h of the step holds q0 + 8 h + 2 kk, q0 + 8 h + 2 kk + 1
+  // loads of wave step t (clamped to the wave's last step: the surplus loads of
+  // the ring's tail re-read it and are never used) into a ring slot; sub-block h
+  // of the step holds embedded indices q0 + 8 h + 2 kk and q0 + 8 h + 2 kk + 1
+  const int64_t lastStep = mySteps > 0 ? mySteps - 1 : 0;
   auto load = [&](int64_t t, auto& sa, auto& sb) {
-    const int64_t q0 = kBegin + (wave + t * kWaves) * static_cast<int64_t>(kStep);
+    const int64_t q0 = kBegin + (wave + min(t, lastStep) * kWaves) * static_cast<int64_t>(kStep);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int64_t q = q0 + 8 * h + 2 * kk;
-      const bool inK = q < kEnd;  // q + 1 < kEnd too unless real and Kp odd
+      const bool in0 = q < kEnd, in1 = q + 1 < kEnd;  // complex: in1 == in0 (q, kEnd even)
+      const int64_t qc = in0 ? q : kBegin;            // a valid address for every lane
+      const int64_t qc1 = in1 ? q + 1 : qc;
 #pragma unroll
       for (int f = 0; f < FM; ++f) {
-        dbl2v v = dbl2v{0.0, 0.0};
-        if (va[f] && inK) {
-          if (CPLX) {
-            v = *reinterpret_cast<const dbl2v*>(pa[f] + q);
-          } else {
-            v.x = pa[f][q];
-            if (q + 1 < kEnd) v.y = pa[f][q + 1];
-          }
+        dbl2v v;
+        if (CPLX) {
+          v = *reinterpret_cast<const dbl2v*>(pa[f] + qc);
+        } else {
+          v.x = pa[f][qc];
+          v.y = pa[f][qc1];
         }
-        sa[f][h] = v;
+        sa[f][h] = dbl2v{(va[f] && in0) ? v.x : 0.0, (va[f] && in1) ? v.y : 0.0};
       }
 #pragma unroll
       for (int g = 0; g < FN; ++g) {
-        dbl2v v = dbl2v{0.0, 0.0};
-        if (vb[g] && inK) {
-          if (CPLX) {
-            v = *reinterpret_cast<const dbl2v*>(pb[g] + q);
-          } else {
-            v.x = pb[g][q];
-            if (q + 1 < kEnd) v.y = pb[g][q + 1];
-          }
+        dbl2v v;
+        if (CPLX) {
+          v = *reinterpret_cast<const dbl2v*>(pb[g] + qc);
+        } else {
+          v.x = pb[g][qc];
+          v.y = pb[g][qc1];
         }
-        sb[g][h] = v;
+        sb[g][h] = dbl2v{(vb[g] && in0) ? v.x : 0.0, (vb[g] && in1) ? v.y : 0.0};
       }
     }
   };
@@ -155,15 +157,14 @@ __global__ void __launch_bounds__(kWaves * 64) skinny_gemm_kernel(GemmDesc d) {
     }
   };
 
+  if (mySteps > 0) {
 #pragma unroll
-  for (int u = 0; u < DEPTH; ++u)
-    if (u < mySteps) load(u, ra[u], rb[u]);
-  for (int64_t base = 0; base < mySteps; base += DEPTH) {
+    for (int u = 0; u < DEPTH; ++u) load(u, ra[u], rb[u]);
+    for (int64_t base = 0; base < mySteps; base += DEPTH) {
 #pragma unroll
-    for (int u = 0; u < DEPTH; ++u) {
-      if (base + u < mySteps) {
-        compute(ra[u], rb[u]);
-        if (base + u + DEPTH < mySteps) load(base + u + DEPTH, ra[u], rb[u]);
+      for (int u = 0; u < DEPTH; ++u) {
+        if (base + u < mySteps) compute(ra[u], rb[u]);
+        load(base + u + DEPTH, ra[u], rb[u]);
       }
     }
   }
