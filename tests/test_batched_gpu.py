@@ -225,6 +225,33 @@ def test_batch_split_precision(device):
         check_block(s, h, 2, 1.0 - 1.0j, 0.25)
 
 
+def test_batch_split_precision_sbs_blocks(device):
+    """sbs blocks (A m x k, MN-major) in split precision: residues of an MN-major A."""
+    import torch
+
+    ctx = spla.Context(spla.ProcessingUnit.GPU)
+    ctx.set_gemm_mode(spla.GemmMode.SPLIT_INT8)
+    sizes = [(700, 96, 3000), (300, 200, 5000)]
+    specs, host = [], []
+    for i, (m, n, k) in enumerate(sizes):
+        A = make_buffer(45, 3 * i + 1, m, k, m + 1, np.complex128)
+        B = make_buffer(45, 3 * i + 2, k, n, k, np.complex128)
+        C = make_buffer(45, 3 * i + 3, m, n, m, np.complex128)
+        dA, dB, dC = (to_device(x, device) for x in (A, B, C))
+        specs.append(dict(m=m, n=n, k=k, A=dA, lda=m + 1, B=dB, ldb=k, C=dC, ldc=m))
+        host.append((A, B, C))
+    torch.cuda.synchronize()
+    batch = spla.Batch(ctx, specs, 0.5 + 0.5j, -1.0, opA=spla.Operation.NONE)
+    batch.execute()
+    assert ctx.last_product_info()["split"]
+    for s, (A, B, C) in zip(specs, host):
+        m, n, k = s["m"], s["n"], s["k"]
+        ref = C.copy()
+        oracle.gemm(0, 0, m, n, k, 0.5 + 0.5j, A, s["lda"], B, k, -1.0, ref, m, accurate=True)
+        assert rel_error(s["C"].cpu().numpy(), ref, A, B) <= 1e-13
+    batch.close()
+
+
 def test_batch_rejects_host_pointers(ctx, device):
     A = np.zeros(64, dtype=np.complex128)
     specs, _ = make_blocks(14, [(8, 8, 8)], np.complex128, device)
