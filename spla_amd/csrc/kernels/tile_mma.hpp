@@ -39,7 +39,16 @@ struct Vec<float> {
 };
 
 constexpr int kThreads = 256;
-constexpr int kPad = 2;  // LDS row padding (elements): keeps pair alignment, spreads banks
+// LDS row padding (elements; even, so that element pairs stay 16-byte aligned).
+// Rows of BK = 16 doubles are 128 bytes. Measured on gfx950 with the
+// bank-conflict counter on the fragment reads (ds_read_b128: 16 rows x 4
+// 16-byte chunks per wave): 160-byte rows read conflict-free, 144-, 176- and
+// 208-byte rows cost 4 conflict cycles per read. 160-byte rows cost 8 cycles on
+// an MN-major store (eight rows, one chunk), which is rarer than the reads of
+// the same tile; they fill the 160 KB of a CU exactly with two real 128 x 128
+// or four real 64 x 64 workgroups. FP32 rows keep 2 floats of padding.
+template <typename S, bool CPLX>
+constexpr int kPadOf = sizeof(S) == 8 ? 4 : 2;
 
 __device__ __forceinline__ double flip_sign(double x, uint32_t hiMask) {
   uint64_t b = __builtin_bit_cast(uint64_t, x);
@@ -57,12 +66,21 @@ __device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// 16-byte LDS read of an element pair: the tiles are 16-byte aligned with rows
+// of whole 16-byte multiples, so every pair at an even column is (ds_read_b128).
+template <typename S>
+__device__ __forceinline__ typename Vec<S>::V2 lds_pair(const S* p) {
+  return *reinterpret_cast<const typename Vec<S>::V2*>(
+      __builtin_assume_aligned(p, 2 * sizeof(S)));
+}
+
 // Stages one operand tile (ROWS stored rows x BK embedded reduction elements)
 // from global memory into registers and from registers into LDS.
 // UNCHECKED: k-blocks wholly inside the operands skip the per-element checks
 // (measured faster on the large tiles, slower on the small ones).
 template <typename S, bool CPLX, int LAYOUT, int ROWS, int BK, bool UNCHECKED>
 struct TileStage {
+  static constexpr int kPad = kPadOf<S, CPLX>;
   using V2 = typename Vec<S>::V2;
   static constexpr int kPairs = ROWS * BK / 2;
   static constexpr int kPerThread = kPairs / kThreads;
@@ -172,9 +190,12 @@ struct TileMma {
   __device__ __forceinline__ static int cplx_row(int r) { return kF64 ? (r & 7) : (r >> 1); }
   __device__ __forceinline__ static bool imag_row(int r) { return kF64 ? r >= 8 : (r & 1) != 0; }
 
+  static constexpr int kPad = kPadOf<S, CPLX>;
+  // rows are whole 16-byte multiples; the 16-byte alignment lets the fragment
+  // reads compile to ds_read_b128
   struct Smem {
-    S lA[2][AR][BK + kPad];
-    S lB[2][BN][BK + kPad];
+    alignas(16) S lA[2][AR][BK + kPad];
+    alignas(16) S lB[2][BN][BK + kPad];
   };
 
   __device__ __forceinline__ static void run(const TileOperands<S>& op, int row0, int col0,
@@ -228,7 +249,7 @@ struct TileMma {
 #pragma unroll
         for (int f = 0; f < FM; ++f) {
           const int sr = CPLX ? (wm * WM) / 2 + 8 * f + cplx_row(l16) : wm * WM + 16 * f + l16;
-          const V2 v = *reinterpret_cast<const V2*>(&sm.lA[buf][sr][kb8 * 8 + 2 * kk]);
+          const V2 v = lds_pair(&sm.lA[buf][sr][kb8 * 8 + 2 * kk]);
           if (CPLX) {
             const S x = swapPair ? v.y : v.x;
             const S y = swapPair ? v.x : v.y;
@@ -242,7 +263,7 @@ struct TileMma {
 #pragma unroll
         for (int g = 0; g < FN; ++g) {
           const int j = wn * WN + 16 * g + l16;
-          b[set][g] = *reinterpret_cast<const V2*>(&sm.lB[buf][j][kb8 * 8 + 2 * kk]);
+          b[set][g] = lds_pair(&sm.lB[buf][j][kb8 * 8 + 2 * kk]);
         }
       };
 
@@ -319,7 +340,7 @@ struct TileMma {
 #pragma unroll
           for (int f = 0; f < FM; ++f) {
             const int sr = CPLX ? (wm * WM) / 2 + 8 * f + cplx_row(l16) : wm * WM + 16 * f + l16;
-            const V2 v = *reinterpret_cast<const V2*>(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
+            const V2 v = lds_pair(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
             if (CPLX) {
               const S x = swapPair ? v.y : v.x;
               const S y = swapPair ? v.x : v.y;
@@ -333,7 +354,7 @@ struct TileMma {
 #pragma unroll
           for (int g = 0; g < FN; ++g) {
             const int j = wn * WN + 16 * g + l16;
-            b[g] = *reinterpret_cast<const V2*>(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
+            b[g] = lds_pair(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
           }
 #pragma unroll
           for (int f = 0; f < FM; ++f)
@@ -380,7 +401,7 @@ struct TileMma {
 #pragma unroll
           for (int f = 0; f < FM; ++f) {
             const int sr = CPLX ? (wm * WM) / 2 + 8 * f + cplx_row(l16) : wm * WM + 16 * f + l16;
-            const V2 v = *reinterpret_cast<const V2*>(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
+            const V2 v = lds_pair(&sm.lA[cur][sr][kb8 * 8 + 2 * kk]);
             if (CPLX) {
               const S x = swapPair ? v.y : v.x;
               const S y = swapPair ? v.x : v.y;
@@ -394,7 +415,7 @@ struct TileMma {
 #pragma unroll
           for (int g = 0; g < FN; ++g) {
             const int j = wn * WN + 16 * g + l16;
-            b[g] = *reinterpret_cast<const V2*>(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
+            b[g] = lds_pair(&sm.lB[cur][j][kb8 * 8 + 2 * kk]);
           }
 #pragma unroll
           for (int f = 0; f < FM; ++f)

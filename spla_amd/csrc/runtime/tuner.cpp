@@ -119,9 +119,7 @@ GemmPlan plan_gemm(bool complex, int64_t m, int64_t n, int64_t k, int numCUs, in
   return best;
 }
 
-double Tuner::efficiency(int cfg) const {
-  auto it = eff_.find(cfg);
-  if (it != eff_.end()) return it->second;
+double Tuner::efficiency(int cfg) {
   switch (cfg) {
     case TileXL:
       return 0.85;
@@ -130,12 +128,6 @@ double Tuner::efficiency(int cfg) const {
     default:
       return 0.8;
   }
-}
-
-void Tuner::record_efficiency(int cfg, double eff) {
-  std::lock_guard<std::mutex> guard(lock_);
-  eff_[cfg] = eff;
-  cache_.clear();
 }
 
 GemmPlan Tuner::plan(bool complex, int layoutA, int layoutB, int64_t m, int64_t n, int64_t k,

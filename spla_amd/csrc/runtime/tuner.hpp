@@ -3,8 +3,10 @@
 // A plan is picked by a cost model of the launch: for every tile shape and
 // split factor the model counts waves of workgroups over the CUs, the time of
 // one wave at the shape's sustained MFMA rate, tile padding waste, and the
-// HBM time of the split-K reduction; the cheapest plan wins. Measured rates
-// can be fed back with record_rate(); plans are cached per problem key.
+// HBM time of the split-K reduction; the cheapest plan wins. Plans are cached
+// per problem key and depend only on the shape, so that repeated calls of one
+// product split the reduction alike and return bitwise identical results (the
+// batch API's plans, by contrast, can be timed: see batched.hpp).
 // Environment overrides (for tuning sweeps): SPLA_AMD_TILE=L|M|XL,
 // SPLA_AMD_SPLITS=<n>.
 #pragma once
@@ -33,14 +35,12 @@ public:
                 int numCUs, bool wholeChip = true);
 
   // Sustained fraction of the per-CU FP64 matrix rate assumed for a tile config.
-  double efficiency(int cfg) const;
-  void record_efficiency(int cfg, double eff);
+  static double efficiency(int cfg);
 
 private:
   using Key = std::tuple<bool, int, int, int64_t, int64_t, int64_t, int, bool>;
   std::mutex lock_;
   std::map<Key, GemmPlan> cache_;
-  std::map<int, double> eff_;
 };
 
 // Pure cost model (no state), exposed for tests.
