@@ -145,8 +145,17 @@ SPLA_EXPORT SplaError spla_amd_batch_create_sbs(SplaAmdBatch* batch, SplaContext
                                                 const void* beta, SplaAmdTuning tuning);
 
 /* Runs the batch (synchronous). In split-precision mode every block runs as
- * its own split product. */
+ * its own split product. In automatic mode (SPLA_AMD_GEMM_MODE_AUTO) the blocks
+ * whose shape rules the split form out join one grouped FP64 launch; the others
+ * run split, block by block, unless the sum of their split-form lower bounds
+ * does not beat the grouped launch's estimate for them, in which case every
+ * block runs in the grouped FP64 launch. */
 SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch);
+
+/* Blocks the last spla_amd_batch_execute ran in the split form, block by block
+ * (0: every block in the grouped FP64 launch). No counterpart in the reference,
+ * which has no batch object. */
+SPLA_EXPORT SplaError spla_amd_batch_split_blocks(SplaAmdBatch batch, int* blocks);
 
 /* Plan summary: tile configuration (0: 128 x 128, 1: 64 x 64 embedded), work
  * items, workgroups of the persistent launch, modelled and measured (0 when

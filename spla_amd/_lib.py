@@ -118,6 +118,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("spla_amd_batch_create_sbs", P(c_void_p), c_void_p, c_int, c_int, c_void_p, c_void_p,
        c_void_p, c_int)
     fn("spla_amd_batch_persistent", c_void_p, P(c_int))
+    fn("spla_amd_batch_split_blocks", c_void_p, P(c_int))
     fn("spla_amd_batch_execute", c_void_p)
     fn("spla_amd_zgemm_noalloc", c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
        c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p)

@@ -367,9 +367,11 @@ class Batch:
                                             ctypes.byref(meas)))
         pers = ctypes.c_int()
         check(self._lib.spla_amd_batch_persistent(self._h, ctypes.byref(pers)))
+        split = ctypes.c_int()
+        check(self._lib.spla_amd_batch_split_blocks(self._h, ctypes.byref(split)))
         return {"tile_config": cfg.value, "items": items.value, "workgroups": wgs.value,
                 "model_ms": model.value, "measured_ms": meas.value,
-                "persistent": bool(pers.value)}
+                "persistent": bool(pers.value), "split_blocks": split.value}
 
     def close(self) -> None:
         if self._h:

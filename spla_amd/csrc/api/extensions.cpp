@@ -113,6 +113,13 @@ SPLA_EXPORT SplaError spla_amd_batch_persistent(SplaAmdBatch batch, int* persist
   SPLA_C_GUARD(*persistent = static_cast<spla_amd::BatchPlan*>(batch)->persistent() ? 1 : 0)
 }
 
+SPLA_EXPORT SplaError spla_amd_batch_split_blocks(SplaAmdBatch batch, int* blocks) {
+  if (!batch) return SPLA_INVALID_HANDLE_ERROR;
+  if (!blocks) return SPLA_INVALID_POINTER_ERROR;
+  *blocks = static_cast<const spla_amd::BatchPlan*>(batch)->last_split_blocks();
+  return SPLA_SUCCESS;
+}
+
 SPLA_EXPORT SplaError spla_amd_batch_execute(SplaAmdBatch batch) {
   if (!batch) return SPLA_INVALID_HANDLE_ERROR;
   SPLA_C_GUARD(static_cast<spla_amd::BatchPlan*>(batch)->execute())

@@ -367,27 +367,40 @@ void BatchPlan::execute() {
   ctx_->timer().reset();
   const bool alphaZero = alpha_[0] == 0.0 && alpha_[1] == 0.0;
   const int mode = ctx_->gemm_mode();
-  if (mode == GemmModeFP64 || alphaZero) {
+  // Automatic mode: the blocks whose shape rules the split form out (a lower
+  // bound of its cost with the fewest moduli any data could need, against the
+  // block's own FP64 estimate) stay on FP64; the others are split candidates.
+  // The candidates would otherwise join the grouped launch, so they run split
+  // only when their lower bounds, one after another on the whole chip, beat
+  // the grouped launch's model time for them.
+  std::vector<BlockSpec> fp64, split;
+  std::vector<char> mask;
+  bool allFp64 = mode == GemmModeFP64 || alphaZero;
+  if (!allFp64 && mode == GemmModeAuto) {
+    mask.assign(specs_.size(), 0);
+    double splitSeconds = 0.0;
+    for (std::size_t i = 0; i < specs_.size(); ++i) {
+      const BlockSpec& b = specs_[i];
+      if (b.m == 0 || b.n == 0) continue;
+      double lower = 0.0;
+      mask[i] = may_split(b, &lower) ? 1 : 0;
+      if (mask[i]) splitSeconds += lower;
+      (mask[i] ? split : fp64).push_back(b);
+    }
+    allFp64 = split.empty() || splitSeconds >= 0.8 * best_schedule(split).modelSeconds;
+  }
+  if (allFp64) {
     // the grouped GEMM and the split reduction, replayed from the graph
+    lastSplitBlocks_ = 0;
     PhaseScope ps(ctx_->timer(), PhaseGemmF64, s);
     full_.run(s);
   } else {
-    // Split precision, or the automatic choice. Automatic mode: the blocks
-    // whose shape rules the split form out (a lower bound of its cost with the
-    // fewest moduli any data could need against the FP64 estimate) run as one
-    // grouped FP64 launch on the compute stream; the others, and every block in
+    // Split precision, or the automatic choice: the FP64 blocks run as one
+    // grouped launch on the compute stream; the split blocks, and every block in
     // split mode, go block by block over the context's batch streams, staged so
     // that all of them share one host synchronization for their plans
     // (execute_split_batch; blocks whose plan falls back run the FP64 kernels).
-    std::vector<BlockSpec> fp64, split;
     if (mode == GemmModeAuto) {
-      std::vector<char> mask(specs_.size(), 0);
-      for (std::size_t i = 0; i < specs_.size(); ++i) {
-        const BlockSpec& b = specs_[i];
-        if (b.m == 0 || b.n == 0) continue;
-        mask[i] = may_split(b) ? 1 : 0;
-        (mask[i] ? split : fp64).push_back(b);
-      }
       if (!fp64.empty() && (!autoLaunch_ || mask != autoMask_)) {
         autoLaunch_ = std::make_unique<GroupedLaunch>();
         autoLaunch_->build(*ctx_, complex_, opA_, fp64, alpha_, beta_, best_schedule(fp64),
@@ -397,6 +410,7 @@ void BatchPlan::execute() {
     } else {
       split = specs_;
     }
+    lastSplitBlocks_ = static_cast<int>(split.size());
     // lanes of the block-by-block products: the batch streams, and the compute
     // stream too when no grouped launch occupies it
     constexpr int kMaxLanes = 1 + spla::ContextInternal::kBatchStreams;
@@ -439,7 +453,7 @@ void BatchPlan::execute() {
   if (ctx_->timer().enabled()) ctx_->timer().resolve();
 }
 
-bool BatchPlan::may_split(const BlockSpec& b) const {
+bool BatchPlan::may_split(const BlockSpec& b, double* lowerBound) const {
   const double fp64 = fp64_estimate_seconds(*ctx_, complex_, opA_, SPLA_OP_NONE, b.m, b.n, b.k,
                                             /*wholeChip=*/false);
   if (complex_) {
@@ -447,11 +461,13 @@ bool BatchPlan::may_split(const BlockSpec& b) const {
     SplitProduct<Z> p(opA_, SPLA_OP_NONE, b.m, b.n, b.k, Z(alpha_[0], alpha_[1]),
                       static_cast<const Z*>(b.A), b.lda, static_cast<const Z*>(b.B), b.ldb,
                       Z(beta_[0], beta_[1]), static_cast<Z*>(b.C), b.ldc, *ctx_, FillSpec());
+    *lowerBound = p.lower_bound_seconds();
     return p.may_beat_fp64(fp64);
   }
   SplitProduct<double> p(opA_, SPLA_OP_NONE, b.m, b.n, b.k, alpha_[0],
                          static_cast<const double*>(b.A), b.lda, static_cast<const double*>(b.B),
                          b.ldb, beta_[0], static_cast<double*>(b.C), b.ldc, *ctx_, FillSpec());
+  *lowerBound = p.lower_bound_seconds();
   return p.may_beat_fp64(fp64);
 }
 

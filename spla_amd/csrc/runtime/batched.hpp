@@ -97,6 +97,9 @@ public:
   int workgroups() const { return static_cast<int>(full_.workgroups()); }
   double model_ms() const { return sched_.modelSeconds * 1e3; }
   double measured_ms() const { return measuredMs_; }
+  // Blocks the last execute() ran in the split form, block by block (0: every
+  // block in the grouped FP64 launch).
+  int last_split_blocks() const { return lastSplitBlocks_; }
   bool persistent() const { return full_.persistent(); }
 
 private:
@@ -104,7 +107,9 @@ private:
   BatchSchedule best_schedule(const std::vector<BlockSpec>& specs) const;
   double time_candidate(const BatchSchedule& c, bool persistent);
   // automatic mode: whether the split form could beat FP64 on this block's shape
-  bool may_split(const BlockSpec& b) const;
+  // Whether block b may run faster split than on FP64 (its own estimates), and
+  // the lower bound of its split-form time.
+  bool may_split(const BlockSpec& b, double* lowerBound) const;
 
   std::shared_ptr<spla::ContextInternal> ctx_;
   bool complex_;
@@ -114,6 +119,7 @@ private:
   BatchSchedule sched_;
   GroupedLaunch full_;
   // automatic mode: the grouped launch of the blocks that stay on FP64
+  int lastSplitBlocks_ = 0;
   std::unique_ptr<GroupedLaunch> autoLaunch_;
   std::vector<char> autoMask_;
   double measuredMs_ = 0.0;

@@ -96,11 +96,11 @@ bool SplitProduct<T>::enqueue_stats(hipStream_t stream) {
 }
 
 template <typename T>
-bool SplitProduct<T>::may_beat_fp64(double fp64Seconds) {
+double SplitProduct<T>::lower_bound_seconds() const {
   constexpr bool cplx = Cplx<T>::value;
   constexpr int comps = cplx ? 2 : 1;
   constexpr int products = cplx ? 3 : 1;
-  if (m_ == 0 || n_ == 0 || k_ == 0) return true;
+  if (m_ == 0 || n_ == 0 || k_ == 0) return 0.0;
   const double tol = std::min(ctx_.split_tolerance(), std::max(fp64_bound(k_, comps), 1e-300));
   const double G = static_cast<double>(min_moduli(tol)) * products;
   const double kpad = static_cast<double>(round_up(k_, kKpadAlign));
@@ -109,8 +109,13 @@ bool SplitProduct<T>::may_beat_fp64(double fp64Seconds) {
   const double ops = 2.0 * G * md * nd * kpad;
   const double bytes = (md + nd) * static_cast<double>(k_) * comps * 8.0 * 2 + G * (md + nd) * kpad +
                        G * md * nd;
-  const double est = ops / kInt8OpsPerSecond + bytes / kHbmBytesPerSecond + kSplitOverheadSeconds;
-  if (est < 0.8 * fp64Seconds) return true;
+  return ops / kInt8OpsPerSecond + bytes / kHbmBytesPerSecond + kSplitOverheadSeconds;
+}
+
+template <typename T>
+bool SplitProduct<T>::may_beat_fp64(double fp64Seconds) {
+  if (m_ == 0 || n_ == 0 || k_ == 0) return true;
+  if (lower_bound_seconds() < 0.8 * fp64Seconds) return true;
   rep_.reason = SplitNotFaster;
   return false;
 }

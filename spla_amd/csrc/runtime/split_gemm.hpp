@@ -40,6 +40,9 @@ public:
   // full-chip INT8 rate, cannot make the split form faster than fp64Seconds.
   // plan() would reject every such product, so its statistics pass is skipped.
   bool may_beat_fp64(double fp64Seconds);
+  // The cost may_beat_fp64 compares: a lower bound of the split form's time
+  // (fewest moduli, full-chip INT8 rate, no tile padding, one reduction split).
+  double lower_bound_seconds() const;
   // False when the product falls back to the FP64 kernels (report().reason).
   bool plan(double fp64Seconds);
   // Device bytes of the residue planes and INT8 GEMM output (after plan()).

@@ -208,6 +208,9 @@ def test_batch_parameter_errors_precede_device_work():
     assert lib.spla_amd_batch_create_ssb(ctypes.byref(h), ctx.handle, 1, 2, 1, None, one, one,
                                          0) == spla.SplaError.INVALID_POINTER_ERROR
     assert lib.spla_amd_batch_execute(None) == spla.SplaError.INVALID_HANDLE_ERROR
+    v = ctypes.c_int()
+    assert lib.spla_amd_batch_split_blocks(None, ctypes.byref(v)) == \
+        spla.SplaError.INVALID_HANDLE_ERROR
     assert lib.spla_amd_batch_destroy(None) == spla.SplaError.INVALID_HANDLE_ERROR
 
 

@@ -325,6 +325,8 @@ def test_batch_automatic_mode_mixes_grouped_fp64_and_split(device):
     specs, host = make_blocks(16, sizes, np.complex128, device)
     batch = spla.Batch(ctx, specs, 1.0 - 0.5j, 0.0)
     batch.execute()
+    # the 1024 x 1024 x 40000 block is the one whose split form pays off
+    assert batch.info()["split_blocks"] == 1, batch.info()
     first = [s["C"].cpu().numpy() for s in specs]
     for s, h in zip(specs, host):
         check_block(s, h, 2, 1.0 - 0.5j, 0.0)
@@ -332,3 +334,25 @@ def test_batch_automatic_mode_mixes_grouped_fp64_and_split(device):
     for s, f in zip(specs, first):
         assert np.array_equal(s["C"].cpu().numpy(), f)
     batch.close()
+
+
+def test_batch_automatic_mode_keeps_grouped_launch_when_split_does_not_pay(device):
+    """Automatic mode on small blocks with long reductions: each 64 x 64 x 60000 block
+    may beat its own stand-alone FP64 estimate split (few tiles, long reduction), but
+    together their split lower bounds (1.2 ms) do not beat the grouped FP64 launch's
+    estimate for them (0.2 ms), so every block stays in the grouped launch; the results
+    then equal the FP64 mode's bit for bit."""
+    sizes = [(64, 64, 60000)] * 4 + [(40, 30, 5000), (100, 80, 3000)]
+    results = {}
+    for mode in (spla.GemmMode.AUTO, spla.GemmMode.FP64):
+        ctx = spla.Context(spla.ProcessingUnit.GPU)
+        ctx.set_gemm_mode(mode)
+        specs, host = make_blocks(23, sizes, np.complex128, device)
+        batch = spla.Batch(ctx, specs, 1.0, 0.0)
+        batch.execute()
+        if mode == spla.GemmMode.AUTO:
+            assert batch.info()["split_blocks"] == 0, batch.info()
+        results[mode] = [s["C"].cpu().numpy() for s in specs]
+        batch.close()
+    for a, b in zip(results[spla.GemmMode.AUTO], results[spla.GemmMode.FP64]):
+        assert np.array_equal(a, b)
