@@ -410,7 +410,8 @@ void BatchPlan::execute() {
     } else {
       split = specs_;
     }
-    lastSplitBlocks_ = static_cast<int>(split.size());
+    lastSplitBlocks_ = static_cast<int>(std::count_if(
+        split.begin(), split.end(), [](const BlockSpec& b) { return b.m > 0 && b.n > 0; }));
     // lanes of the block-by-block products: the batch streams, and the compute
     // stream too when no grouped launch occupies it
     constexpr int kMaxLanes = 1 + spla::ContextInternal::kBatchStreams;
