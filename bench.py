@@ -62,7 +62,7 @@ INT8_PEAK_TOPS = 5000.0
 # --copy-calibration ...), each scaled by what the counter reads for a 1 GiB device
 # copy (reads x 2.0, writes x 1.0). They change only when the kernels or the split
 # plan (number of moduli) change.
-HBM_TRAFFIC_BYTES = {"split": 51.66e9 + 1.41e9, "plain": 116.67e9 + 0.07e9}
+HBM_TRAFFIC_BYTES = {"split": 51.66e9 + 1.48e9, "plain": 116.67e9 + 0.07e9}
 PROFILED_MODULI = 14  # moduli of the split plan the counters were taken on
 METRIC = "FP64-accurate TFLOP/s, zgemm A^H·B k=2e5 m=n=2048: split-prec vs plain FP64"
 
@@ -869,7 +869,9 @@ def main() -> int:
         ops = 2.0 * m * n * k * 3 * best["moduli"]
         gemm_ms = best["phase_ms_per_step"].get("split_int8_gemm", float("nan"))
         achieved = ops / (gemm_ms / 1e3) / 1e12
-        roofline = {"bound": "mfma", "kernel": "i8_gemm_kernel (v_mfma_i32_32x32x32_i8)",
+        form = ("v_mfma_i32_32x32x32_i8" if os.environ.get("SPLA_AMD_I8_MFMA") == "32"
+                else "v_mfma_i32_16x16x64_i8")
+        roofline = {"bound": "mfma", "kernel": f"i8_gemm_kernel ({form})",
                     "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                     "frac": round(achieved / INT8_PEAK_TOPS, 4),
                     # counters were taken on the plan with PROFILED_MODULI moduli

@@ -1,6 +1,6 @@
 // Split-precision (Ozaki scheme II) kernels for gfx950: FP64-accurate products
-// on the INT8 matrix cores (v_mfma_i32_32x32x32_i8). See ozaki_desc.hpp for
-// the arithmetic.
+// on the INT8 matrix cores (v_mfma_i32_16x16x64_i8, or v_mfma_i32_32x32x32_i8
+// with SPLA_AMD_I8_MFMA=32). See ozaki_desc.hpp for the arithmetic.
 //
 //   stats_*      per-row largest |component|, sum of squares, non-finite flag
 //                (HBM bound: reads each operand once);
@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 
 #include "kernels/crt_arith.hpp"
 #include "kernels/launch.hpp"
@@ -334,14 +335,51 @@ __device__ __forceinline__ bool tile_active(int mode, int64_t rowOff, int64_t co
 // k-block), and every memory instruction of a step (global loads, LDS writes of
 // the next k-block, LDS operand reads of the next 32-byte step) is interleaved
 // with its 64 MFMAs.
-template <bool OUT8, int WM, int WN>
+// Accumulators of one MFMA: 32 x 32 (16 registers) or 16 x 16 (4 registers).
+template <bool M16>
+struct I8Acc {
+  typedef int16v type;
+  static constexpr int regs = 16;
+};
+template <>
+struct I8Acc<true> {
+  typedef int4v type;
+  static constexpr int regs = 4;
+};
+
+// v_mfma_i32_16x16x64_i8 with the accumulator tied to its AGPRs. The compiler
+// has no tied (accumulate-in-place) form of 16 x 16 MFMAs, and with several
+// updates of every accumulator per loop iteration its register allocator
+// rotates the 64 accumulators of a 128 x 128 wave tile through the register
+// file (hundreds of AGPR moves per k-block). Issued from inline assembly the
+// accumulator stays in place; as a volatile statement it also keeps its place
+// among the memory instructions, so the issue order is the source order.
+// Hazards the compiler no longer sees: the operands come from LDS reads (the
+// compiler waits for them), an accumulator is updated again only 63 MFMAs
+// later, and the accumulators are read back only after explicit wait states.
+__device__ __forceinline__ void i8_mfma16(int4v& c, const int4v& a, const int4v& b) {
+  asm volatile("v_mfma_i32_16x16x64_i8 %0, %1, %2, %0" : "+a"(c) : "v"(a), "v"(b));
+}
+
+// M16 selects v_mfma_i32_16x16x64_i8 over v_mfma_i32_32x32x32_i8. Both do the
+// same work per matrix-core cycle, but on random residue bytes a register-only
+// loop of the 16 x 16 x 64 form sustains ~4070 TOP/s against ~3430 for the
+// 32 x 32 x 32 form under the same power limit (constant bytes: ~4950 both):
+// it touches half the accumulator bits per multiply-add.
+template <bool OUT8, int WM, int WN, bool M16>
 __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   constexpr int BM = 2 * WM;
   constexpr int BN = 2 * WN;
   constexpr int BK = 128;
-  constexpr int kStride = BK + 16;
-  constexpr int FM = WM / 32;
-  constexpr int FN = WN / 32;
+  // LDS row stride: the 32 x 32 fragments (32 rows x 2 chunks of 16 bytes per
+  // wave read) are conflict-free at 144 bytes, the 16 x 16 ones (16 rows x 4
+  // chunks) only at 160 bytes (bank-conflict counter: 144 bytes cost ~2.7
+  // cycles per LDS instruction); 2 x 512 rows x 160 bytes is the whole 160 KiB
+  constexpr int kStride = BK + (M16 ? 32 : 16);
+  constexpr int FM = WM / (M16 ? 16 : 32);
+  constexpr int FN = WN / (M16 ? 16 : 32);
+  typedef typename I8Acc<M16>::type AccV;
+  constexpr int kAccRegs = I8Acc<M16>::regs;
   constexpr int CA = BM / 32;  // 16-byte chunks per thread and k-block (32 rows per pass)
   constexpr int CB = BN / 32;
   __shared__ __attribute__((aligned(16))) int8_t lA[2][BM][kStride];
@@ -391,14 +429,16 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   const int wn = wid >> 1;
   const int l32 = lane & 31;
   const int khalf = lane >> 5;
+  const int l16 = lane & 15;
+  const int kq = lane >> 4;
 
-  int16v acc[FM][FN];
+  AccV acc[FM][FN];
 #pragma unroll
   for (int f = 0; f < FM; ++f)
 #pragma unroll
     for (int q = 0; q < FN; ++q)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[f][q][r] = 0;
+      for (int r = 0; r < kAccRegs; ++r) acc[f][q][r] = 0;
 
   int4v ra0[CA], rb0[CB], ra1[CA], rb1[CB];
   // k-block index kb is clamped to the last one: the surplus loads of the
@@ -434,6 +474,7 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   // The first sub-step's fragments are read right after the barrier.
   auto step = [&](int buf, int4v (&la)[CA], int4v (&lb)[CB], int kb, const int4v (&sa)[CA],
                   const int4v (&sb)[CB], int sbuf) {
+   if constexpr (!M16) {
     constexpr int kSub = BK / 32;          // sub-steps of 32 bytes
     constexpr int kMfma = FM * FN;         // MFMAs per sub-step
     constexpr int kReads = FM + FN;        // LDS fragment reads per sub-step
@@ -508,11 +549,183 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
           __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
       }
     }
+   } else {
+    // 16 x 16 x 64 MFMAs: two sub-steps of 64 bytes. Sub-step 0 runs by
+    // fragment rows of A, sub-step 1 by fragment columns of B, so one register
+    // set per operand suffices: A fragment h takes its sub-step-1 bytes once
+    // row h is issued, B fragment q once the last row has used it, and
+    // sub-step 1 reads nothing from LDS. The MFMAs are volatile inline
+    // assembly, so the memory instructions issue where the source puts them:
+    //   * before the first MFMA: A fragment 0, then every B fragment (each
+    //     MFMA of row 0 waits only for its own B fragment);
+    //   * sub-step 0: A fragment q + 1 after MFMA (0, q); after MFMA (h, 0) the
+    //     sub-step-1 read of A fragment h - 1; in the last row the sub-step-1
+    //     read of B fragment q after MFMA (h, q), and the last A read after the
+    //     last MFMA; two global loads of k-block kb into (la, lb) per row;
+    //   * sub-step 1: the LDS writes of register set (sa, sb) into buffer sbuf,
+    //     one per MFMA, ending a quarter of the sub-step (large tiles) or four
+    //     MFMAs before the barrier.
+    constexpr int F = FM;
+    constexpr int kMfma = F * F;
+    constexpr int kLoads = CA + CB;
+    constexpr int kWrites = CA + CB;
+    constexpr int kWriteStart = F >= 8 ? kMfma - kMfma / 4 - kWrites : kMfma - 4 - kWrites;
+    static_assert(FM == FN && CA == CB && BK == 128 && kLoads == 2 * F && F % 4 == 0 &&
+                      kWriteStart >= 0,
+                  "tile schedule");
+    const int64_t k0 = kBegin + static_cast<int64_t>(min(kb, nk - 1)) * BK;
+    const int8_t* Ak = At + k0;
+    const int8_t* Bk = Bt + k0;
+    auto fragA = [&](int f, int ks) {
+      return *reinterpret_cast<const int4v*>(&lA[buf][wm * WM + 16 * f + l16][ks * 64 + 16 * kq]);
+    };
+    auto fragB = [&](int q, int ks) {
+      return *reinterpret_cast<const int4v*>(&lB[buf][wn * WN + 16 * q + l16][ks * 64 + 16 * kq]);
+    };
+    // global load index after MFMA (h, q) of sub-step 0, or -1: two per row
+    auto loadIndex = [](int h, int q) {
+      return (q % (F / 2) == F / 2 - 1) ? 2 * h + q / (F / 2) : -1;
+    };
+    int4v a[F], b[F];
+    a[0] = fragA(0, 0);
+#pragma unroll
+    for (int q = 0; q < F; ++q) b[q] = fragB(q, 0);
+#pragma unroll
+    for (int h = 0; h < F; ++h) {
+#pragma unroll
+      for (int q = 0; q < F; ++q) {
+        i8_mfma16(acc[h][q], a[h], b[q]);
+        if (h == 0 && q + 1 < F) a[q + 1] = fragA(q + 1, 0);
+        if (q == 0 && h >= 1) a[h - 1] = fragA(h - 1, 1);
+        if (h == F - 1) b[q] = fragB(q, 1);
+        const int gi = loadIndex(h, q);
+        if (gi >= 0) {
+          if (gi < CA)
+            la[gi] = *reinterpret_cast<const int4v*>(Ak + voff[gi]);
+          else
+            lb[gi - CA] = *reinterpret_cast<const int4v*>(Bk + voff[gi - CA]);
+        }
+      }
+    }
+    a[F - 1] = fragA(F - 1, 1);
+#pragma unroll
+    for (int q = 0; q < F; ++q) {
+#pragma unroll
+      for (int h = 0; h < F; ++h) {
+        i8_mfma16(acc[h][q], a[h], b[q]);
+        const int u = q * F + h - kWriteStart;
+        if (u >= 0 && u < kWrites) {
+          if (u < CA)
+            *reinterpret_cast<int4v*>(&lA[sbuf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = sa[u];
+          else
+            *reinterpret_cast<int4v*>(&lB[sbuf][(tid >> 3) + 32 * (u - CA)][16 * (tid & 7)]) =
+                sb[u - CA];
+        }
+      }
+    }
+   }
   };
 
   // nk is even (splits are whole multiples of 2 * BK) and positive for every
   // launched block; the test only guards the address arithmetic
-  if (nk > 0) {
+  if constexpr (M16) {
+    // Every launched block has a positive, even number of k-blocks (splits are
+    // whole multiples of 2 * BK), so the loop runs and the accumulators reach
+    // the epilogue without a join with an untouched path (a join would make
+    // the compiler move them before the wait states); the test only guards the
+    // address arithmetic.
+    if (nk <= 0) return;
+    load(ra0, rb0, 0);
+    store(ra0, rb0, 0);
+    if constexpr (FM >= 8) {
+      // Large tiles: the LDS serves ~94 B/clk per CU of the ~128 it can, so its
+      // reads and writes are spread evenly over the MFMAs. Fragment set X
+      // holds a k-block's sub-step-0 fragments, set Y its sub-step-1 ones;
+      // one register set (ra0, rb0) stages the k-block after next. One step,
+      // k-block kb in LDS buffer buf (X already read):
+      //   * sub-step 0 (MFMAs on X): Y read from buf, one fragment after every
+      //     4th MFMA; the staged k-block kb + 1 written to buf ^ 1 (first
+      //     half of it), each register reloaded with k-block kb + 2 four
+      //     MFMAs after its write;
+      //   * sub-step 1 (MFMAs on Y): the second half written and reloaded over
+      //     the first 16 MFMAs, then the one barrier of the k-block, then X
+      //     read from buf ^ 1 (k-block kb + 1) after every third MFMA.
+      // Every read of buf precedes the barrier of its k-block and every write
+      // of buf follows it, so one barrier per k-block orders both buffers.
+      int4v xa[FM], xb[FN], ya[FM], yb[FN];
+      // fragment j of a set in the order the row-major MFMAs consume them:
+      // A 0, B 0..F-1, A 1..F-1
+      auto readSet = [&](int4v (&sa_)[FM], int4v (&sb_)[FN], int j, int lbuf, int ks) {
+        const int8_t* baseA = &lA[lbuf][wm * WM + l16][ks * 64 + 16 * kq];
+        const int8_t* baseB = &lB[lbuf][wn * WN + l16][ks * 64 + 16 * kq];
+        if (j == 0)
+          sa_[0] = *reinterpret_cast<const int4v*>(baseA);
+        else if (j <= FN)
+          sb_[j - 1] = *reinterpret_cast<const int4v*>(baseB + (j - 1) * 16 * kStride);
+        else
+          sa_[j - FN] = *reinterpret_cast<const int4v*>(baseA + (j - FN) * 16 * kStride);
+      };
+      auto writeStaged = [&](int u, int sbuf) {
+        if (u < CA)
+          *reinterpret_cast<int4v*>(&lA[sbuf][(tid >> 3) + 32 * u][16 * (tid & 7)]) = ra0[u];
+        else
+          *reinterpret_cast<int4v*>(&lB[sbuf][(tid >> 3) + 32 * (u - CA)][16 * (tid & 7)]) =
+              rb0[u - CA];
+      };
+      auto loadStaged = [&](int u, int kb) {
+        const int64_t k0 = kBegin + static_cast<int64_t>(min(kb, nk - 1)) * BK;
+        if (u < CA)
+          ra0[u] = *reinterpret_cast<const int4v*>(At + k0 + voff[u]);
+        else
+          rb0[u - CA] = *reinterpret_cast<const int4v*>(Bt + k0 + voff[u - CA]);
+      };
+      constexpr int kSet = FM + FN;  // fragments per set
+      static_assert(FM == FN && CA == CB && CA + CB == 16 && kSet == 16 && FM * FN == 64,
+                    "large-tile schedule");
+      auto stepF = [&](int buf, int kb) {
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+          i8_mfma16(acc[i / FN][i % FN], xa[i / FN], xb[i % FN]);
+          if (i % 4 == 0) readSet(ya, yb, i / 4, buf, 1);
+          if (i % 8 == 2) writeStaged(i / 8, buf ^ 1);
+          if (i % 8 == 6) loadStaged(i / 8, kb + 2);
+        }
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+          i8_mfma16(acc[i / FN][i % FN], ya[i / FN], yb[i % FN]);
+          if (i < 16 && i % 2 == 0) writeStaged(8 + i / 2, buf ^ 1);
+          if (i < 16 && i % 2 == 1) loadStaged(8 + i / 2, kb + 2);
+          if (i == 15) __syncthreads();
+          if (i >= 16 && (i - 16) % 3 == 0) readSet(xa, xb, (i - 16) / 3, buf ^ 1, 0);
+        }
+      };
+      load(ra0, rb0, 1);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kSet; ++j) readSet(xa, xb, j, 0, 0);
+      for (int it = 0; it < nk; it += 2) {
+        stepF(0, it);
+        stepF(1, it + 1);
+      }
+    } else {
+      load(ra1, rb1, 1);
+      __syncthreads();
+      for (int it = 0; it < nk; it += 2) {
+        step(0, ra0, rb0, it + 2, ra1, rb1, 1);
+        __syncthreads();
+        step(1, ra1, rb1, it + 3, ra0, rb0, 0);
+        __syncthreads();
+      }
+    }
+    // the last MFMAs (inline assembly, 8 passes) finish before any accumulator
+    // is read or moved: the wait states come first among the volatile
+    // statements, and every later read depends on one of the empty ones
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+#pragma unroll
+    for (int f = 0; f < FM; ++f)
+#pragma unroll
+      for (int q = 0; q < FN; ++q) asm volatile("" : "+a"(acc[f][q]));
+  } else if (nk > 0) {
     load(ra0, rb0, 0);
     store(ra0, rb0, 0);
     load(ra1, rb1, 1);
@@ -529,21 +742,24 @@ __global__ void __launch_bounds__(256, 1) i8_gemm_kernel(I8GemmDesc d) {
   }
 
   // D layout of v_mfma_i32_32x32x32_i8: lane holds column lane % 32 and rows
-  // 8 * (r / 4) + 4 * (lane / 32) + r % 4, r = 0..15: four consecutive rows per
-  // group of four registers, stored as one word (biased residues) or one
-  // 16-byte vector (int32 sums). Output columns hold ldo >= every tile row.
+  // 8 * (r / 4) + 4 * (lane / 32) + r % 4, r = 0..15; of v_mfma_i32_16x16x64_i8:
+  // column lane % 16 and rows 4 * (lane / 16) + r, r = 0..3. Four consecutive
+  // rows per group of four registers, stored as one word (biased residues) or
+  // one 16-byte vector (int32 sums). Output columns hold ldo >= every tile row.
   const int tmod = g / d.products;
   const double dm = d.ms.m[tmod], dinv = d.ms.inv[tmod];
   const double tie = d.ms.kc[tmod].tie;  // m = 256: ties go up, r in [-128, 127]
+  constexpr int kFrag = M16 ? 16 : 32;
 #pragma unroll
   for (int f = 0; f < FM; ++f) {
 #pragma unroll
     for (int q = 0; q < FN; ++q) {
-      const int col = col0 + wn * WN + 32 * q + l32;
+      const int col = col0 + wn * WN + kFrag * q + (M16 ? l16 : l32);
       if (col >= d.n) continue;
 #pragma unroll
-      for (int rb4 = 0; rb4 < 4; ++rb4) {
-        const int row = row0 + wm * WM + 32 * f + 8 * rb4 + 4 * khalf;
+      for (int rb4 = 0; rb4 < kAccRegs / 4; ++rb4) {
+        const int row = M16 ? row0 + wm * WM + 16 * f + 4 * kq
+                            : row0 + wm * WM + 32 * f + 8 * rb4 + 4 * khalf;
         if (OUT8) {
           uint32_t w = 0;
 #pragma unroll
@@ -782,19 +998,42 @@ hipError_t launch_residues(bool complex, int layout, const ResidueDesc& d, hipSt
   return hipGetLastError();
 }
 
+// SPLA_AMD_I8_MFMA=32 selects the 32 x 32 x 32 form of the INT8 GEMM (bit-identical
+// results, ~10 % slower on the 2048 x 2048 x 2e5 complex product); the default is
+// the 16 x 16 x 64 form. Read at every launch.
+static bool i8_mfma16() {
+  const char* v = std::getenv("SPLA_AMD_I8_MFMA");
+  return !(v && std::atoi(v) == 32);
+}
+
 hipError_t launch_i8_gemm(const I8GemmDesc& d, int cfg, hipStream_t stream) {
   const unsigned blocks = static_cast<unsigned>(d.tilesM) * d.tilesN * d.G * d.splits;
   const bool out8 = d.splits == 1;
+  const bool m16 = i8_mfma16();
   if (cfg == I8TileM) {
-    if (out8)
-      hipLaunchKernelGGL((i8_gemm_kernel<true, 64, 64>), dim3(blocks), dim3(256), 0, stream, d);
-    else
-      hipLaunchKernelGGL((i8_gemm_kernel<false, 64, 64>), dim3(blocks), dim3(256), 0, stream, d);
+    if (m16) {
+      if (out8)
+        hipLaunchKernelGGL((i8_gemm_kernel<true, 64, 64, true>), dim3(blocks), dim3(256), 0, stream, d);
+      else
+        hipLaunchKernelGGL((i8_gemm_kernel<false, 64, 64, true>), dim3(blocks), dim3(256), 0, stream, d);
+    } else {
+      if (out8)
+        hipLaunchKernelGGL((i8_gemm_kernel<true, 64, 64, false>), dim3(blocks), dim3(256), 0, stream, d);
+      else
+        hipLaunchKernelGGL((i8_gemm_kernel<false, 64, 64, false>), dim3(blocks), dim3(256), 0, stream, d);
+    }
   } else {
-    if (out8)
-      hipLaunchKernelGGL((i8_gemm_kernel<true, 128, 128>), dim3(blocks), dim3(256), 0, stream, d);
-    else
-      hipLaunchKernelGGL((i8_gemm_kernel<false, 128, 128>), dim3(blocks), dim3(256), 0, stream, d);
+    if (m16) {
+      if (out8)
+        hipLaunchKernelGGL((i8_gemm_kernel<true, 128, 128, true>), dim3(blocks), dim3(256), 0, stream, d);
+      else
+        hipLaunchKernelGGL((i8_gemm_kernel<false, 128, 128, true>), dim3(blocks), dim3(256), 0, stream, d);
+    } else {
+      if (out8)
+        hipLaunchKernelGGL((i8_gemm_kernel<true, 128, 128, false>), dim3(blocks), dim3(256), 0, stream, d);
+      else
+        hipLaunchKernelGGL((i8_gemm_kernel<false, 128, 128, false>), dim3(blocks), dim3(256), 0, stream, d);
+    }
   }
   return hipGetLastError();
 }

@@ -132,6 +132,37 @@ def test_split_large_tiles_bit_exact(sctx, device, case):
     assert all(g == w for g, w in zip(got, want)), case
 
 
+@pytest.mark.parametrize("case", ["z_large_out8", "d_large_out8", "z_small_int32", "d_small_out8"])
+def test_int8_gemm_mfma_forms_bit_identical(sctx, device, monkeypatch, case):
+    """The INT8 GEMM's two matrix-core forms (16 x 16 x 64, the default, and
+    32 x 32 x 32, SPLA_AMD_I8_MFMA=32) give the same product bit for bit over the
+    whole output: 256 x 256 tiles with uint8 residue output, and 128 x 128 tiles
+    with uint8 output and with INT32 split sums (k > 2^17 - 256)."""
+    import torch
+
+    cplx = case.startswith("z")
+    m, n, k = {"z_large_out8": (1024, 1024, 1500), "d_large_out8": (1792, 1536, 700),
+               "z_small_int32": (72, 40, 140000), "d_small_out8": (130, 70, 3000)}[case]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(97)
+    dtype = torch.complex128 if cplx else torch.float64
+
+    def operand(rows):
+        x = torch.rand(rows * k * (2 if cplx else 1), generator=gen, device=device,
+                       dtype=torch.float64) * 2 - 1
+        return torch.view_as_complex(x.view(-1, 2)) if cplx else x
+
+    dA, dB = operand(m), operand(n)
+    out = {}
+    for form in ("16", "32"):
+        monkeypatch.setenv("SPLA_AMD_I8_MFMA", form)
+        dC = torch.zeros(m * n, dtype=dtype, device=device)
+        spla.gemm(2 if cplx else 1, 0, m, n, k, 1.0, dA, k, dB, k, 0.0, dC, m, sctx)
+        assert sctx.last_product_info()["split"]
+        out[form] = dC.cpu().numpy()
+    assert np.array_equal(out["16"], out["32"]), case
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
 def test_split_k_beyond_int32_range(sctx, device, dtype):
     m, n, k = 40, 24, 140000  # > 2^17 - 128: INT32 sums are split
