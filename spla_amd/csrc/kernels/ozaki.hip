@@ -56,12 +56,31 @@ __global__ void __launch_bounds__(256) stats_kmajor_kernel(const double* __restr
   const double* row = X + static_cast<int64_t>(r) * ld * (CPLX ? 2 : 1);
   double mx = 0.0, sq = 0.0;
   int bad = 0;
-  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    const double x = row[q];
+  auto take = [&](double x) {
     bad |= !isfinite(x);
     mx = fmax(mx, fabs(x));
     sq += x * x;
+  };
+  // 16-byte non-temporal loads (the operand is read once more, by the residue
+  // pass, long after it has left the caches): one leading element when the range
+  // starts at an odd double, pairs, one trailing element
+  const double* p = row + q0;
+  int64_t cnt = q1 - q0;
+  if (cnt > 0 && (reinterpret_cast<uintptr_t>(p) & 15) != 0) {
+    if (threadIdx.x == 0) take(p[0]);
+    ++p;
+    --cnt;
   }
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const d2v* p2 = reinterpret_cast<const d2v*>(p);
+  const int64_t pairs = cnt / 2;
+#pragma unroll 4
+  for (int64_t j = threadIdx.x; j < pairs; j += 256) {
+    const d2v v = __builtin_nontemporal_load(p2 + j);
+    take(v.x);
+    take(v.y);
+  }
+  if ((cnt & 1) != 0 && threadIdx.x == 0) take(p[cnt - 1]);
   __shared__ double smx[256], ssq[256];
   __shared__ int sbad[256];
   smx[threadIdx.x] = mx;

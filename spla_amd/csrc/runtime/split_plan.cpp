@@ -18,6 +18,25 @@ double fp64_bound(int64_t k, int comps) {
 
 namespace {
 
+// ilogb of a positive finite double: the exponent field for normal numbers (the
+// library for subnormals). The planner runs between the statistics pass and the
+// residue pass, so its per-row work is kept to bit operations.
+int ilogb_pos(double x) {
+  uint64_t b;
+  std::memcpy(&b, &x, sizeof b);
+  const int e = static_cast<int>((b >> 52) & 0x7ff);
+  return e != 0 ? e - 1023 : std::ilogb(x);
+}
+
+// 2^e exactly, built from the bits in the normal range
+double pow2(int e) {
+  if (e < -1022 || e > 1023) return std::ldexp(1.0, e);
+  const uint64_t b = static_cast<uint64_t>(e + 1023) << 52;
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
 struct RowScan {
   bool finite = true;
   bool zero = true;
@@ -35,7 +54,7 @@ RowScan scan(const OperandStats& s) {
     }
     if (mx > 0.0) {
       r.zero = false;
-      r.maxExp = std::max(r.maxExp, std::ilogb(mx));
+      r.maxExp = std::max(r.maxExp, ilogb_pos(mx));
     }
     r.normSq += s.sumsq[i];
   }
@@ -46,7 +65,7 @@ RowScan scan(const OperandStats& s) {
 double scale_factor(const OperandStats& s, const RowScan& r) {
   double acc = 0.0;
   for (int i = 0; i < s.rows; ++i)
-    if (s.maxabs[i] > 0.0) acc += std::ldexp(1.0, 2 * (std::ilogb(s.maxabs[i]) - r.maxExp));
+    if (s.maxabs[i] > 0.0) acc += pow2(2 * (ilogb_pos(s.maxabs[i]) - r.maxExp));
   const double comps_k = static_cast<double>(s.comps) * static_cast<double>(s.k);
   // sqrt(comps k sum_i 4^fl_i) / ||X|| with 4^fl_i = 4^maxExp * 4^(fl_i - maxExp)
   return std::ldexp(std::sqrt(comps_k * acc), r.maxExp) / std::sqrt(r.normSq);
@@ -55,27 +74,39 @@ double scale_factor(const OperandStats& s, const RowScan& r) {
 void exponents(const OperandStats& s, int P, std::vector<int>& e) {
   e.assign(s.rows, 0);
   for (int i = 0; i < s.rows; ++i)
-    if (s.maxabs[i] > 0.0) e[i] = P - 1 - std::ilogb(s.maxabs[i]);
+    if (s.maxabs[i] > 0.0) e[i] = P - 1 - ilogb_pos(s.maxabs[i]);
 }
 
-// log2 of an upper bound of the Euclidean norm of the scaled integer rows.
+// log2 of an upper bound of the Euclidean norm of the scaled integer rows:
+// max_i min(cap, log2(2^e_i ||x_i|| + 0.5 sqrt(comps k))). The estimate grows with
+// 2^(2 e_i) sumsq_i, so only the row where that is largest (compared exactly, as
+// 2 e_i + exponent and significand of sumsq_i) is evaluated; a row with a positive
+// maximum but a zero sum of squares (underflow) takes the cap, the largest value.
 double max_row_norm_log2(const OperandStats& s, const std::vector<int>& e, int P) {
   const double comps_k = static_cast<double>(s.comps) * static_cast<double>(s.k);
   const double capLog2 = P + 0.5 * std::log2(comps_k);  // every |a| <= 2^P
-  double best = -std::numeric_limits<double>::infinity();
+  int best = -1;
+  int64_t bestKey = 0;
+  double bestSig = 0.0;
   for (int i = 0; i < s.rows; ++i) {
     if (!(s.maxabs[i] > 0.0)) continue;
-    double est = capLog2;
-    if (s.sumsq[i] > 0.0) {
-      // 2^e ||x|| + 0.5 sqrt(comps k), in log2
-      const double a = e[i] + 0.5 * std::log2(s.sumsq[i]);
-      const double b = std::log2(0.5 * std::sqrt(comps_k));
-      const double hi = std::max(a, b), lo = std::min(a, b);
-      est = std::min(capLog2, hi + std::log2(1.0 + std::exp2(lo - hi)));
+    const double sq = s.sumsq[i];
+    if (!(sq > 0.0)) return capLog2;
+    int ex;
+    const double sig = std::frexp(sq, &ex);  // sq = sig 2^ex, sig in [0.5, 1)
+    const int64_t key = 2 * static_cast<int64_t>(e[i]) + ex;
+    if (best < 0 || key > bestKey || (key == bestKey && sig > bestSig)) {
+      best = i;
+      bestKey = key;
+      bestSig = sig;
     }
-    best = std::max(best, est);
   }
-  return best;
+  if (best < 0) return -std::numeric_limits<double>::infinity();
+  // 2^e ||x|| + 0.5 sqrt(comps k), in log2
+  const double a = e[best] + 0.5 * std::log2(s.sumsq[best]);
+  const double b = std::log2(0.5 * std::sqrt(comps_k));
+  const double hi = std::max(a, b), lo = std::min(a, b);
+  return std::min(capLog2, hi + std::log2(1.0 + std::exp2(lo - hi)));
 }
 
 }  // namespace
