@@ -96,67 +96,77 @@ __global__ void __launch_bounds__(kWaves * 64, 2) skinny_gemm_kernel(GemmDesc d)
 #pragma unroll
     for (int g = 0; g < FN; ++g) acc[f][g] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-  // loads of wave step t (clamped to the wave's last step: the surplus loads of
-  // the ring's tail re-read it and are never used) into a ring slot; sub-block h
-  // of the step holds embedded indices q0 + 8 h + 2 kk and q0 + 8 h + 2 kk + 1
-  const int64_t lastStep = mySteps > 0 ? mySteps - 1 : 0;
+  // loads of wave step t into a ring slot; sub-block h of the step holds
+  // embedded indices q0 + 8 h + 2 kk and q0 + 8 h + 2 kk + 1. Indices past kEnd
+  // (the range's tail, and the steps past the wave's last one that the ring's
+  // tail loads) read the range's first element, rows past m and columns past n
+  // read row / column 0; compute() zeroes all of them where it uses the values
+  // (zeroing right after the load would make the wave wait for each load), so
+  // the ring runs without a branch and the surplus steps add zeros.
+  auto step_q = [&](int64_t t, int h) {
+    return kBegin + (wave + t * kWaves) * static_cast<int64_t>(kStep) + 8 * h + 2 * kk;
+  };
   auto load = [&](int64_t t, auto& sa, auto& sb) {
-    const int64_t q0 = kBegin + (wave + min(t, lastStep) * kWaves) * static_cast<int64_t>(kStep);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int64_t q = q0 + 8 * h + 2 * kk;
+      const int64_t q = step_q(t, h);
       const bool in0 = q < kEnd, in1 = q + 1 < kEnd;  // complex: in1 == in0 (q, kEnd even)
       const int64_t qc = in0 ? q : kBegin;            // a valid address for every lane
       const int64_t qc1 = in1 ? q + 1 : qc;
 #pragma unroll
       for (int f = 0; f < FM; ++f) {
-        dbl2v v;
         if (CPLX) {
-          v = *reinterpret_cast<const dbl2v*>(pa[f] + qc);
+          sa[f][h] = *reinterpret_cast<const dbl2v*>(pa[f] + qc);
         } else {
-          v.x = pa[f][qc];
-          v.y = pa[f][qc1];
+          sa[f][h].x = pa[f][qc];
+          sa[f][h].y = pa[f][qc1];
         }
-        sa[f][h] = dbl2v{(va[f] && in0) ? v.x : 0.0, (va[f] && in1) ? v.y : 0.0};
       }
 #pragma unroll
       for (int g = 0; g < FN; ++g) {
-        dbl2v v;
         if (CPLX) {
-          v = *reinterpret_cast<const dbl2v*>(pb[g] + qc);
+          sb[g][h] = *reinterpret_cast<const dbl2v*>(pb[g] + qc);
         } else {
-          v.x = pb[g][qc];
-          v.y = pb[g][qc1];
+          sb[g][h].x = pb[g][qc];
+          sb[g][h].y = pb[g][qc1];
         }
-        sb[g][h] = dbl2v{(vb[g] && in0) ? v.x : 0.0, (vb[g] && in1) ? v.y : 0.0};
       }
     }
   };
-  auto compute = [&](const auto& sa, const auto& sb) {
+  auto compute = [&](int64_t t, const auto& sa, const auto& sb) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      double a0[FM], a1[FM];
+      const int64_t q = step_q(t, h);
+      const bool in0 = q < kEnd, in1 = q + 1 < kEnd;
+      double a0[FM], a1[FM], b0[FN], b1[FN];
 #pragma unroll
       for (int f = 0; f < FM; ++f) {
         const dbl2v v = sa[f][h];
+        const double vx = (va[f] && in0) ? v.x : 0.0;
+        const double vy = (va[f] && in1) ? v.y : 0.0;
         if (CPLX) {
-          const double x = imagRow ? v.y : v.x;
-          const double y = imagRow ? v.x : v.y;
+          const double x = imagRow ? vy : vx;
+          const double y = imagRow ? vx : vy;
           a0[f] = flip_sign(x, sgn0);
           a1[f] = flip_sign(y, sgn1);
         } else {
-          a0[f] = v.x;
-          a1[f] = v.y;
+          a0[f] = vx;
+          a1[f] = vy;
         }
+      }
+#pragma unroll
+      for (int g = 0; g < FN; ++g) {
+        b0[g] = (vb[g] && in0) ? sb[g][h].x : 0.0;
+        b1[g] = (vb[g] && in1) ? sb[g][h].y : 0.0;
       }
 #pragma unroll
       for (int f = 0; f < FM; ++f)
 #pragma unroll
-        for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a0[f], sb[g][h].x, acc[f][g]);
+        for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a0[f], b0[g], acc[f][g]);
 #pragma unroll
       for (int f = 0; f < FM; ++f)
 #pragma unroll
-        for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a1[f], sb[g][h].y, acc[f][g]);
+        for (int g = 0; g < FN; ++g) acc[f][g] = mfma16x16x4(a1[f], b1[g], acc[f][g]);
     }
   };
 
@@ -166,7 +176,7 @@ __global__ void __launch_bounds__(kWaves * 64, 2) skinny_gemm_kernel(GemmDesc d)
     for (int64_t base = 0; base < mySteps; base += DEPTH) {
 #pragma unroll
       for (int u = 0; u < DEPTH; ++u) {
-        if (base + u < mySteps) compute(ra[u], rb[u]);
+        compute(base + u, ra[u], rb[u]);
         load(base + u + DEPTH, ra[u], rb[u]);
       }
     }
