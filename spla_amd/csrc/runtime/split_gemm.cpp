@@ -17,18 +17,11 @@
 namespace spla_amd {
 namespace {
 
-// Residue rows and reduction splits are whole multiples of two GEMM k-blocks
-// (2 x 128 bytes): the GEMM pipeline runs k-blocks in pairs.
-constexpr int64_t kKpadAlign = 256;
-// Largest split of the reduction (bytes of int8 residues) whose INT32 sums
-// cannot overflow: 128 * 128 * (2^17 - 256) < 2^31.
-constexpr int64_t kMaxSplitK = (int64_t(1) << 17) - 256;
 // longest padded residue row: offsets of the 256 rows of a tile fit 32 bits
 constexpr int64_t kMaxKpad = int64_t(1) << 24;
 // residue planes hold whole tiles of the INT8 GEMM (rows padded to 256)
 constexpr int64_t kPlaneRowAlign = 256;
 // Sustained rates assumed by the automatic mode.
-constexpr double kInt8OpsPerSecond = 2.0e15;  // 256 x 256 tiles, full chip
 constexpr double kHbmBytesPerSecond = 4.0e12;
 // statistics round trip to the host, plan and the six launches of a product
 constexpr double kSplitOverheadSeconds = 150e-6;
@@ -144,34 +137,15 @@ bool SplitProduct<T>::plan(double fp64Seconds) {
   kpad_ = round_up(k_, kKpadAlign);
   mpad_ = round_up(m_, kPlaneRowAlign);
   npad_ = round_up(n_, kPlaneRowAlign);
-  // INT8 GEMM shape: large tiles unless they leave the chip half empty; the
-  // reduction is split to stay within the INT32 range and to fill the chip
+  // INT8 GEMM tile shape and reduction splits
   const int numCUs = ctx_.num_cus();
-  cfg_ = I8TileL;
-  i8_tile_shape(cfg_, bm_, bn_);
-  int64_t tiles = ((m_ + bm_ - 1) / bm_) * ((n_ + bn_ - 1) / bn_);
-  if (tiles * G_ < 2 * numCUs) {
-    cfg_ = I8TileM;
-    i8_tile_shape(cfg_, bm_, bn_);
-    tiles = ((m_ + bm_ - 1) / bm_) * ((n_ + bn_ - 1) / bn_);
-  }
-  splits_ = (kpad_ + kMaxSplitK - 1) / kMaxSplitK;
-  while (tiles * G_ * splits_ < 2 * numCUs && kpad_ / (splits_ + 1) >= 2048) ++splits_;
-  if (splits_ > 1) {
-    // One workgroup per CU: the GEMM takes ceil(blocks / CUs) waves of kpad / splits
-    // each. Already summing INT32 splits, up to three more splits are taken when they
-    // fill the last wave better (e.g. 15 products x 64 tiles: 2 splits leave the 8th
-    // wave half empty, 4 splits make exactly 15 full waves).
-    auto cost = [&](int64_t sp) {
-      return std::ceil(static_cast<double>(tiles * G_ * sp) / numCUs) / static_cast<double>(sp);
-    };
-    int64_t best = splits_;
-    for (int64_t sp = splits_ + 1; sp <= splits_ + 3 && kpad_ / sp >= 2048; ++sp)
-      if (cost(sp) < 0.97 * cost(best)) best = sp;
-    splits_ = best;
-  }
-  kSplit_ = round_up((kpad_ + splits_ - 1) / splits_, kKpadAlign);
-  splits_ = (kpad_ + kSplit_ - 1) / kSplit_;
+  const I8Shape sh = plan_i8_shape(m_, n_, kpad_, static_cast<int>(G_), numCUs);
+  cfg_ = sh.cfg;
+  bm_ = sh.bm;
+  bn_ = sh.bn;
+  splits_ = sh.splits;
+  kSplit_ = sh.kSplit;
+  const int64_t tiles = sh.tiles;
   if (ctx_.gemm_mode() == GemmModeAuto) {
     // INT8 work on whole tiles at the rate of the chosen tile, scaled by how
     // well the workgroups fill the chip (last partial wave), plus the passes
@@ -181,7 +155,7 @@ bool SplitProduct<T>::plan(double fp64Seconds) {
     const double fill = blocks / (waves * numCUs);
     const double ops = 2.0 * G_ * static_cast<double>(((m_ + bm_ - 1) / bm_) * bm_) *
                        static_cast<double>(((n_ + bn_ - 1) / bn_) * bn_) * kpad_;
-    const double rate = kInt8OpsPerSecond * (cfg_ == I8TileL ? 1.0 : 0.6) * fill;
+    const double rate = kInt8OpsPerSecond * (cfg_ == I8TileL ? 1.0 : 0.58) * fill;
     const double bytes = static_cast<double>(m_ + n_) * k_ * comps * 8.0 * 2 +
                          static_cast<double>(G_) * (m_ + n_) * kpad_ +
                          static_cast<double>(G_) * m_ * n_ * (splits_ == 1 ? 1.0 : 4.0 * splits_);

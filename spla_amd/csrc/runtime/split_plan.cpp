@@ -163,6 +163,63 @@ SplitPlan plan_split(const OperandStats& A, const OperandStats& B, double tolera
   return p;
 }
 
+namespace {
+
+int64_t round_up_to(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// HBM rate of the INT32 split sums (written by the GEMM, read by the CRT)
+constexpr double kSumBytesPerSecond = 4.0e12;
+
+I8Shape i8_shape(int cfg, int64_t m, int64_t n, int64_t kpad, int G, int numCUs) {
+  I8Shape sh;
+  sh.cfg = cfg;
+  i8_tile_shape(cfg, sh.bm, sh.bn);
+  sh.tiles = ((m + sh.bm - 1) / sh.bm) * ((n + sh.bn - 1) / sh.bn);
+  // one 256 x 256 workgroup per CU, two 128 x 128 ones
+  const int64_t slots = static_cast<int64_t>(numCUs) * (cfg == I8TileL ? 1 : 2);
+  int64_t splits = (kpad + kMaxSplitK - 1) / kMaxSplitK;
+  while (sh.tiles * G * splits < 2 * numCUs && kpad / (splits + 1) >= 2048) ++splits;
+  if (splits > 1) {
+    // The GEMM takes ceil(blocks / slots) waves of kpad / splits each. Already
+    // summing INT32 splits, up to three more splits are taken when they fill the
+    // last wave better (e.g. 15 products x 64 tiles: 2 splits leave the 8th wave
+    // half empty, 4 splits make exactly 15 full waves).
+    auto cost = [&](int64_t sp) {
+      return std::ceil(static_cast<double>(sh.tiles * G * sp) / slots) / static_cast<double>(sp);
+    };
+    int64_t best = splits;
+    for (int64_t sp = splits + 1; sp <= splits + 3 && kpad / sp >= 2048; ++sp)
+      if (cost(sp) < 0.97 * cost(best)) best = sp;
+    splits = best;
+  }
+  sh.kSplit = round_up_to((kpad + splits - 1) / splits, kKpadAlign);
+  sh.splits = (kpad + sh.kSplit - 1) / sh.kSplit;
+  // whole tiles at the shape's rate (128 x 128 tiles read twice the LDS bytes per
+  // MFMA: ~0.58 of the 256 x 256 rate), the last wave's fill, ~6.5 k-blocks of
+  // prologue and epilogue per workgroup, and the INT32 split sums
+  const double blocks = static_cast<double>(sh.tiles) * G * sh.splits;
+  const double fill = blocks / (std::ceil(blocks / slots) * slots);
+  const double nk = static_cast<double>(sh.kSplit) / 128.0;
+  const double ops = 2.0 * G * static_cast<double>(((m + sh.bm - 1) / sh.bm) * sh.bm) *
+                     static_cast<double>(((n + sh.bn - 1) / sh.bn) * sh.bn) *
+                     static_cast<double>(kpad);
+  const double rate = kInt8OpsPerSecond * (cfg == I8TileL ? 1.0 : 0.58);
+  const double sums = static_cast<double>(G) * m * n * (sh.splits == 1 ? 1.0 : 8.0 * sh.splits);
+  sh.seconds = ops / (rate * fill * nk / (nk + 6.5)) + sums / kSumBytesPerSecond;
+  return sh;
+}
+
+}  // namespace
+
+I8Shape plan_i8_shape(int64_t m, int64_t n, int64_t kpad, int G, int numCUs) {
+  // More splits of 256 x 256 tiles beat 128 x 128 tiles on mid-size products
+  // (512 x 512 x 2e5 complex: 1.98 against 2.9 ms); 128 x 128 tiles win where
+  // 256-row padding would multiply the work.
+  const I8Shape large = i8_shape(I8TileL, m, n, kpad, G, numCUs);
+  const I8Shape medium = i8_shape(I8TileM, m, n, kpad, G, numCUs);
+  return medium.seconds < large.seconds ? medium : large;
+}
+
 int min_moduli(double tolerance) {
   // scale_factor: ||X||^2 < comps k sum_i 4^(fl_i + 1), so S > 1/2 and
   // P = ceil(log2(S / (tol / 2.02))) >= ceil(log2(1.01 / tol)); bitsNeeded is

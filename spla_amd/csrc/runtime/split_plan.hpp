@@ -73,6 +73,27 @@ SplitPlan plan_split(const OperandStats& A, const OperandStats& B, double tolera
 // ceil(log2(1.01 / tolerance)) bits and the moduli must cover twice that.
 int min_moduli(double tolerance);
 
+// Residue rows and reduction splits are whole multiples of two GEMM k-blocks
+// (2 x 128 bytes): the GEMM pipeline runs k-blocks in pairs.
+constexpr int64_t kKpadAlign = 256;
+// Largest split of the reduction (bytes of int8 residues) whose INT32 sums
+// cannot overflow: 128 * 128 * (2^17 - 256) < 2^31.
+constexpr int64_t kMaxSplitK = (int64_t(1) << 17) - 256;
+// Sustained INT8 GEMM rate of 256 x 256 tiles on a full chip, as the cost
+// model assumes it.
+constexpr double kInt8OpsPerSecond = 2.0e15;
+
+// Tile shape (I8TileConfig) and reduction splits of the batched INT8 GEMM of
+// one product: m x n outputs, kpad padded reduction bytes, G products.
+struct I8Shape {
+  int cfg = 0, bm = 0, bn = 0;
+  int64_t tiles = 0, splits = 0, kSplit = 0;
+  double seconds = 0;  // modelled GEMM time
+};
+// For each tile shape the reduction is split to stay within the INT32 range
+// and to fill the chip; the shape with the shorter modelled time is taken.
+I8Shape plan_i8_shape(int64_t m, int64_t n, int64_t kpad, int G, int numCUs);
+
 void make_moduli(int count, ModuliSet& ms);
 void make_crt_constants(const ModuliSet& ms, CrtConstants& cc);
 

@@ -29,6 +29,18 @@ void plan_c(const double* maxA, const double* sumA, int rowsA, const double* max
 
 int min_moduli_c(double tol) { return spla_amd::min_moduli(tol); }
 
+// out: [cfg, bm, bn, tiles, splits, kSplit]; returns the modelled seconds
+double i8_shape_c(long long m, long long n, long long kpad, int G, int numCUs, long long* out) {
+  const spla_amd::I8Shape s = spla_amd::plan_i8_shape(m, n, kpad, G, numCUs);
+  out[0] = s.cfg;
+  out[1] = s.bm;
+  out[2] = s.bn;
+  out[3] = s.tiles;
+  out[4] = s.splits;
+  out[5] = s.kSplit;
+  return s.seconds;
+}
+
 // W: count x 3 limbs, M: 3 limbs, halfM: 3 limbs
 void crt_c(int count, unsigned long long* W, unsigned long long* M, unsigned long long* halfM,
            double* Md) {

@@ -40,6 +40,8 @@ def planner(tmp_path_factory):
     lib.plan_c.argtypes = [d, d, i, d, d, i, ctypes.c_longlong, i, ctypes.c_double, i, d, d, d, d]
     lib.crt_c.argtypes = [i, d, d, d, d]
     lib.min_moduli_c.argtypes = [ctypes.c_double]
+    lib.i8_shape_c.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, i, i, d]
+    lib.i8_shape_c.restype = ctypes.c_double
     return lib
 
 
@@ -221,3 +223,43 @@ def test_emulate_pairs_matches_emulate(cplx):
     pairs = [(i, j) for i in range(m) for j in range(n)]
     got = ozaki.emulate_pairs([opa[i] for i, _ in pairs], [opbT[j] for _, j in pairs], 37, 44)
     assert all(got[s] == C[i, j] for s, (i, j) in enumerate(pairs))
+
+
+def i8_shape(lib, m, n, k, G, cus=256):
+    kpad = (k + 255) // 256 * 256
+    out = np.zeros(6, dtype=np.int64)
+    sec = lib.i8_shape_c(m, n, kpad, G, cus, out.ctypes.data)
+    return dict(cfg=int(out[0]), bm=int(out[1]), bn=int(out[2]), tiles=int(out[3]),
+                splits=int(out[4]), kSplit=int(out[5]), kpad=kpad, seconds=sec)
+
+
+@pytest.mark.parametrize("m,n,k,G,cfg,splits", [
+    (2048, 2048, 200000, 42, 0, 2),   # headline: 256 x 256 tiles, 21 full waves
+    (2048, 2048, 200000, 14, 0, 2),   # real product, 14 moduli: 7 full waves
+    (2048, 2048, 200000, 15, 0, 4),   # real product, 15 moduli: 4 splits fill the last wave
+    (100000, 4096, 4096, 39, 0, 1),   # sbs100k: one reduction split, uint8 output
+    (512, 512, 200000, 42, 0, 6),     # mid-size: more splits of 256 x 256 tiles
+    (1024, 512, 100000, 42, 0, 3),    # 336 tiles: 3 splits fill 4 waves (2 would leave 1.5 of 3)
+    (100, 90, 50000, 42, 1, 16),      # small: 128 x 128 tiles (256-row padding would be 4x),
+                                      # 13 splits reach 2 x 256 blocks, 16 fill two waves
+])
+def test_int8_shape_choice(planner, m, n, k, G, cfg, splits):
+    s = i8_shape(planner, m, n, k, G)
+    assert (s["cfg"], s["splits"]) == (cfg, splits), s
+    assert s["bm"] == s["bn"] == (256 if cfg == 0 else 128)
+    assert s["tiles"] == -(-m // s["bm"]) * -(-n // s["bn"])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_int8_shape_invariants(planner, seed):
+    """Every shape the planner returns covers the reduction with whole 256-byte
+    splits within the INT32 range, and is the cheaper of the two modelled shapes."""
+    rng = np.random.default_rng(seed)
+    for _ in range(200):
+        m, n = (int(x) for x in rng.integers(1, 5000, size=2))
+        k = int(rng.integers(1, 2_000_000))
+        G = int(rng.choice([1, 3])) * int(rng.integers(8, 21))
+        s = i8_shape(planner, m, n, k, G)
+        assert s["kSplit"] % 256 == 0 and 0 < s["kSplit"] <= (1 << 17) - 256
+        assert s["splits"] * s["kSplit"] >= s["kpad"] > (s["splits"] - 1) * s["kSplit"]
+        assert s["seconds"] > 0
