@@ -171,7 +171,8 @@ struct TileOperands {
 // One BM x BN tile of the embedded product over the embedded reduction range
 // [kBegin, kEnd), accumulated in registers. Shared by the single-product
 // kernels and the persistent batched kernel. On return every thread has passed
-// the final barrier, so the LDS buffers may be reused at once.
+// the final barrier, so the LDS buffers may be reused at once (reads that one
+// schedule issues after that barrier only fill registers that are discarded).
 template <typename S, bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK>
 struct TileMma {
   using V2 = typename Vec<S>::V2;
@@ -240,7 +241,10 @@ struct TileMma {
     // fragments are read from LDS while the current one's MFMAs run (two register
     // sets; 2.6 % faster on the complex A^H B tiles). Measured slower elsewhere: 6 %
     // on FP32, 0.7 % with MN-major A (A B), 0.5 % on the small ring-step tiles.
-    constexpr bool kPrefetch = kF64 && LA == KMajor && WM * WN >= 64 * 64;
+    // Real tiles with MN-major B read just in time: with the second set they spilled
+    // 132 VGPRs (2048 x 2048 x 1e5 A^T B^T 44.7 TFLOP/s, 59.6 without).
+    constexpr bool kPrefetch =
+        kF64 && LA == KMajor && WM * WN >= 64 * 64 && (CPLX || LB == KMajor);
     if constexpr (kPrefetch) {
       static_assert((BK / 8) % 2 == 0, "sub-blocks alternate between two register sets");
       S a0[2][FM], a1[2][FM];
@@ -267,6 +271,16 @@ struct TileMma {
         }
       };
 
+      // half = 0: the even reduction elements of a sub-block, 1: the odd ones
+      auto mfma_half = [&](int set, int half) {
+#pragma unroll
+        for (int f = 0; f < FM; ++f)
+#pragma unroll
+          for (int g = 0; g < FN; ++g)
+            acc[f][g] = half ? mfma16x16x4(a1[set][f], b[set][g].y, acc[f][g])
+                             : mfma16x16x4(a0[set][f], b[set][g].x, acc[f][g]);
+      };
+
       if (nk > 0) {
         stA.load(op.A, op.lda, arow0, op.m, kBegin, kEnd, tid);
         stB.load(op.B, op.ldb, col0, op.n, kBegin, kEnd, tid);
@@ -275,6 +289,61 @@ struct TileMma {
       }
       __syncthreads();
       if (nk > 0) read_frags(0, 0, 0);
+
+      // 64 x 64 wave tiles (two waves per SIMD) except real MN-major B, whose staging
+      // leaves no registers for it; the 128 x 64 tiles spill with it.
+      constexpr bool kSplitBarrier =
+          BK == 16 && WM * WN == 64 * 64 && (CPLX || LB == KMajor);
+      if constexpr (kSplitBarrier) {
+        // A k-block's 4 FM FN MFMAs in quarters: (sub-block 0, even elements) with the
+        // reads of sub-block 1; the staged stores into the other buffer; (sub-block 0,
+        // odd), which covers the stores' completion; the barrier; the next k-block's
+        // sub-block 0 reads; both quarters of sub-block 1, which cover those reads.
+        // Against one barrier after all the MFMAs of a k-block, with the stores and
+        // reads on either side of it: 2048 x 2048 x 2e5 complex A^H B 109.8 -> 102.7 ms.
+        if (nk > 1) {
+          stA.load(op.A, op.lda, arow0, op.m, kBegin + BK, kEnd, tid);
+          stB.load(op.B, op.ldb, col0, op.n, kBegin + BK, kEnd, tid);
+        }
+        // k-block it + 2 into the staging registers once they are stored. Real tiles
+        // (eight staged pairs per thread) issue these loads right after the stores;
+        // complex tiles (six) after the barrier and the next fragment reads, which
+        // measured 1-2 % faster for each.
+        // The stores and the reads after the barrier are unconditional (branches here
+        // make the wait counts conservative): in the last k-block they store the
+        // staged registers' last k-block once more into the buffer that is no longer
+        // read, and read them back into registers that are not used (reads issued
+        // after the final barrier whose values are discarded).
+        for (int it = 0; it < nk; ++it) {
+          const int cur = it & 1;
+          read_frags(1, cur, 1);
+          mfma_half(0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          stA.store(sm.lA[cur ^ 1], tid);
+          stB.store(sm.lB[cur ^ 1], tid);
+          if constexpr (!CPLX) {
+            if (it + 2 < nk) {
+              const int64_t q0 = kBegin + static_cast<int64_t>(it + 2) * BK;
+              stA.load(op.A, op.lda, arow0, op.m, q0, kEnd, tid);
+              stB.load(op.B, op.ldb, col0, op.n, q0, kEnd, tid);
+            }
+          }
+          mfma_half(0, 1);
+          __builtin_amdgcn_sched_barrier(0);  // these MFMAs cover the stores' completion
+          __syncthreads();
+          read_frags(0, cur ^ 1, 0);
+          if constexpr (CPLX) {
+            if (it + 2 < nk) {
+              const int64_t q0 = kBegin + static_cast<int64_t>(it + 2) * BK;
+              stA.load(op.A, op.lda, arow0, op.m, q0, kEnd, tid);
+              stB.load(op.B, op.ldb, col0, op.n, q0, kEnd, tid);
+            }
+          }
+          mfma_half(1, 0);
+          mfma_half(1, 1);
+        }
+        return;
+      }
 
       for (int it = 0; it < nk; ++it) {
         const int cur = it & 1;
