@@ -50,11 +50,11 @@ __device__ __forceinline__ void update_c(const BatchDesc& d, const BatchBlock& b
 
 // One (block, tile, split) item; on return every thread has passed the tile
 // routine's last barrier, so the LDS buffers may be reused at once.
-template <bool CPLX, int LA, int WM, int WN>
+template <bool CPLX, int LA, int WM, int WN, bool LEAN>
 __device__ __forceinline__ void batched_item(
     const BatchDesc& d, const BatchItem& it,
     typename TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>::Smem& sm) {
-  using T = TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>;
+  using T = TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16, LEAN>;
   const BatchBlock b = d.blocks[it.block];
   const int row0 = it.tm * T::BM;  // embedded rows
   const int col0 = it.tn * T::BN;
@@ -78,7 +78,7 @@ __device__ __forceinline__ void batched_item(
 template <bool CPLX, int LA, int WM, int WN>
 __global__ void __launch_bounds__(kThreads, 2) batched_gemm_kernel(BatchDesc d) {
   __shared__ typename TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>::Smem sm;
-  batched_item<CPLX, LA, WM, WN>(d, d.items[blockIdx.x], sm);
+  batched_item<CPLX, LA, WM, WN, false>(d, d.items[blockIdx.x], sm);
 }
 
 // Persistent launch: one workgroup per slot works through its static list of
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kThreads, 2) batched_gemm_persistent_kernel(Ba
   __shared__ typename TileMma<double, CPLX, LA, KMajor, WM, WN, 2, 16>::Smem sm;
   const int end = d.slotStart[blockIdx.x + 1];
   for (int i = d.slotStart[blockIdx.x]; i < end; ++i)
-    batched_item<CPLX, LA, WM, WN>(d, d.items[d.slotItems[i]], sm);
+    batched_item<CPLX, LA, WM, WN, true>(d, d.items[d.slotItems[i]], sm);
 }
 
 // AR x BN: element rows and columns of a tile of the GEMM launch.

@@ -168,12 +168,25 @@ struct TileOperands {
   int conjA, conjB;
 };
 
+// LDS of one tile: two buffers of AR stored A rows and BN B rows of BK elements.
+// Rows are whole 16-byte multiples; the 16-byte alignment lets the fragment
+// reads compile to ds_read_b128.
+template <typename S, bool CPLX, int AR, int BN, int BK>
+struct TileSmem {
+  alignas(16) S lA[2][AR][BK + kPadOf<S, CPLX>];
+  alignas(16) S lB[2][BN][BK + kPadOf<S, CPLX>];
+};
+
 // One BM x BN tile of the embedded product over the embedded reduction range
 // [kBegin, kEnd), accumulated in registers. Shared by the single-product
-// kernels and the persistent batched kernel. On return every thread has passed
-// the final barrier, so the LDS buffers may be reused at once (reads that one
-// schedule issues after that barrier only fill registers that are discarded).
-template <typename S, bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK>
+// kernels and the batched kernels. On return every thread has passed the final
+// barrier, so the LDS buffers may be reused at once (reads that one schedule
+// issues after that barrier only fill registers that are discarded).
+// LEAN: for callers that keep more state live across tiles (the persistent
+// batched kernel), which only take the schedules with one fragment set and one
+// barrier per k-block.
+template <typename S, bool CPLX, int LA, int LB, int WM, int WN, int WAVES_M, int BK,
+          bool LEAN = false>
 struct TileMma {
   using V2 = typename Vec<S>::V2;
   using Acc = typename Vec<S>::V4;
@@ -192,12 +205,7 @@ struct TileMma {
   __device__ __forceinline__ static bool imag_row(int r) { return kF64 ? r >= 8 : (r & 1) != 0; }
 
   static constexpr int kPad = kPadOf<S, CPLX>;
-  // rows are whole 16-byte multiples; the 16-byte alignment lets the fragment
-  // reads compile to ds_read_b128
-  struct Smem {
-    alignas(16) S lA[2][AR][BK + kPad];
-    alignas(16) S lB[2][BN][BK + kPad];
-  };
+  using Smem = TileSmem<S, CPLX, AR, BN, BK>;
 
   __device__ __forceinline__ static void run(const TileOperands<S>& op, int row0, int col0,
                                              int64_t kBegin, int64_t kEnd, Smem& sm,
@@ -237,14 +245,17 @@ struct TileMma {
     TileStage<S, CPLX, LA, AR, BK, kUnchecked> stA;
     TileStage<S, CPLX, LB, BN, BK, kUnchecked> stB;
 
-    // FP64 with K-major A on the large tiles: the next 8-element sub-block's
-    // fragments are read from LDS while the current one's MFMAs run (two register
-    // sets; 2.6 % faster on the complex A^H B tiles). Measured slower elsewhere: 6 %
-    // on FP32, 0.7 % with MN-major A (A B), 0.5 % on the small ring-step tiles.
-    // Real tiles with MN-major B read just in time: with the second set they spilled
-    // 132 VGPRs (2048 x 2048 x 1e5 A^T B^T 44.7 TFLOP/s, 59.6 without).
-    constexpr bool kPrefetch =
-        kF64 && LA == KMajor && WM * WN >= 64 * 64 && (CPLX || LB == KMajor);
+    // FP64 on the large tiles: the next 8-element sub-block's fragments are read
+    // from LDS while the current one's MFMAs run (two register sets; 2.6 % faster on
+    // the complex A^H B tiles with one barrier per k-block). Measured slower
+    // elsewhere: 6 % on FP32, 0.5 % on the small ring-step tiles. MN-major A (A B)
+    // takes it on the complex 64 x 64 tiles only, where it carries the split-barrier
+    // schedule below (pzgemm_sbs 100000 x 4096 x 4096 63.6 -> 66.6 TFLOP/s; 0.7 %
+    // slower with one barrier per k-block); the real ones spill with it. Real tiles
+    // with MN-major B read just in time: with the second set they spilled 132 VGPRs
+    // (2048 x 2048 x 1e5 A^T B^T 44.7 TFLOP/s, 59.6 without).
+    constexpr bool kPrefetch = kF64 && WM * WN >= 64 * 64 && (CPLX || LB == KMajor) &&
+                               (LA == KMajor || (!LEAN && CPLX && WM * WN == 64 * 64));
     if constexpr (kPrefetch) {
       static_assert((BK / 8) % 2 == 0, "sub-blocks alternate between two register sets");
       S a0[2][FM], a1[2][FM];
@@ -293,7 +304,7 @@ struct TileMma {
       // 64 x 64 wave tiles (two waves per SIMD) except real MN-major B, whose staging
       // leaves no registers for it; the 128 x 64 tiles spill with it.
       constexpr bool kSplitBarrier =
-          BK == 16 && WM * WN == 64 * 64 && (CPLX || LB == KMajor);
+          !LEAN && BK == 16 && WM * WN == 64 * 64 && (CPLX || LB == KMajor);
       if constexpr (kSplitBarrier) {
         // A k-block's 4 FM FN MFMAs in quarters: (sub-block 0, even elements) with the
         // reads of sub-block 1; the staged stores into the other buffer; (sub-block 0,
