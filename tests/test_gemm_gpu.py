@@ -77,6 +77,28 @@ def test_gemm_integer_inputs_bit_exact(ctx, device, dtype, opA, opB):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
+@pytest.mark.parametrize("tile", ["L", "XL"])
+@pytest.mark.parametrize("opA,opB", [("C", "N"), ("N", "N"), ("T", "T"), ("N", "T")])
+def test_large_tiles_every_kblock_count_bit_exact(ctx, device, dtype, tile, opA, opB,
+                                                  monkeypatch):
+    """The 128 x 128 (L) and 256 x 128 (XL) tiles, forced, on products of 1 to 5 k-blocks
+    of 16 embedded reduction elements in one and in two splits: the first and last
+    k-blocks of each schedule (stores and reads around the barrier, the loads of the
+    k-block after next), every operand layout (K-major or MN-major A and B) and edge
+    tiles in m and n. Integer inputs: bit-exact against the oracle."""
+    monkeypatch.setenv("SPLA_AMD_TILE", tile)
+    per_kblock = 8 if dtype == np.complex128 else 16
+    for splits in ("1", "2"):
+        monkeypatch.setenv("SPLA_AMD_SPLITS", splits)
+        for kb in (1, 2, 3, 4, 5):
+            for m, n in ((130, 140), (64, 128)):
+                k = kb * per_kblock
+                _, _, _, got, ref, _, _, _ = run_case(ctx, device, dtype, opA, opB, m, n, k,
+                                                      kind="int")
+                assert np.array_equal(got, ref), (splits, kb, m, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128], ids=["d", "z"])
 def test_beta_zero_does_not_read_c(ctx, device, dtype):
     m, n, k = 40, 33, 70
     A = make_buffer(3, 1, k, m, k, dtype)
