@@ -245,17 +245,21 @@ struct TileMma {
     TileStage<S, CPLX, LA, AR, BK, kUnchecked> stA;
     TileStage<S, CPLX, LB, BN, BK, kUnchecked> stB;
 
-    // FP64 on the large tiles: the next 8-element sub-block's fragments are read
-    // from LDS while the current one's MFMAs run (two register sets; 2.6 % faster on
-    // the complex A^H B tiles with one barrier per k-block). Measured slower
-    // elsewhere: 6 % on FP32, 0.5 % on the small ring-step tiles. MN-major A (A B)
-    // takes it on the complex 64 x 64 tiles only, where it carries the split-barrier
-    // schedule below (pzgemm_sbs 100000 x 4096 x 4096 63.6 -> 66.6 TFLOP/s; 0.7 %
-    // slower with one barrier per k-block); the real ones spill with it. Real tiles
-    // with MN-major B read just in time: with the second set they spilled 132 VGPRs
-    // (2048 x 2048 x 1e5 A^T B^T 44.7 TFLOP/s, 59.6 without).
-    constexpr bool kPrefetch = kF64 && WM * WN >= 64 * 64 && (CPLX || LB == KMajor) &&
-                               (LA == KMajor || (!LEAN && CPLX && WM * WN == 64 * 64));
+    // Large tiles: the next 8-element sub-block's fragments are read from LDS while
+    // the current one's MFMAs run (two register sets; 2.6 % faster on the complex
+    // FP64 A^H B tiles with one barrier per k-block, but 6 % slower on FP32 and 0.5 %
+    // on the small ring-step tiles). With the split-barrier schedule below the 64 x 64
+    // wave tiles gain more: FP64 MN-major A (A B) takes it on complex tiles only
+    // (pzgemm_sbs 100000 x 4096 x 4096 63.6 -> 66.6 TFLOP/s; the real ones spill);
+    // FP32 in every layout but real MN-major B, at two waves per SIMD (2048 x 2048 x
+    // 1e5: complex A^H B 110.7 -> 120.3 TFLOP/s, real A^T B 111.6 -> 121.5, real A B
+    // 103.4 -> 118.5; three waves per SIMD, 168 VGPRs: 109.6). Real tiles with
+    // MN-major B read just in time: with the second set they spilled 132 VGPRs in
+    // FP64 (2048 x 2048 x 1e5 A^T B^T 44.7 TFLOP/s, 59.6 without).
+    constexpr bool kPrefetch =
+        (kF64 && WM * WN >= 64 * 64 && (CPLX || LB == KMajor) &&
+         (LA == KMajor || (!LEAN && CPLX && WM * WN == 64 * 64))) ||
+        (!kF64 && !LEAN && BK == 16 && WM * WN == 64 * 64 && (CPLX || LB == KMajor));
     if constexpr (kPrefetch) {
       static_assert((BK / 8) % 2 == 0, "sub-blocks alternate between two register sets");
       S a0[2][FM], a1[2][FM];

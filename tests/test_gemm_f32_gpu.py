@@ -72,6 +72,27 @@ def test_f32_integer_inputs_bit_exact(ctx, device, dtype, opA, opB):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64], ids=["s", "c"])
+@pytest.mark.parametrize("opA,opB", [("C", "N"), ("N", "N"), ("T", "T"), ("N", "T")])
+def test_f32_large_tile_every_kblock_count_bit_exact(ctx, device, dtype, opA, opB,
+                                                     monkeypatch):
+    """The 128 x 128 tile (split-barrier schedule in every layout but real MN-major B),
+    forced, on 1 to 5 k-blocks of 16 embedded reduction elements in one and in two
+    splits, edge tiles in m and n; integer inputs, bit-exact."""
+    if dtype == np.float32:
+        opA = "T" if opA == "C" else opA
+    monkeypatch.setenv("SPLA_AMD_TILE", "L")
+    per_kblock = 8 if dtype == np.complex64 else 16
+    for splits in ("1", "2"):
+        monkeypatch.setenv("SPLA_AMD_SPLITS", splits)
+        for kb in (1, 2, 3, 4, 5):
+            for m, n in ((130, 140), (64, 128)):
+                k = kb * per_kblock
+                _, _, got, ref, _, _, ldc = case(ctx, device, dtype, opA, opB, m, n, k, "int")
+                g, r = to_matrix(got, m, n, ldc), to_matrix(ref, m, n, ldc)
+                assert np.array_equal(g.astype(r.dtype), r), (splits, kb, m, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64], ids=["s", "c"])
 @pytest.mark.parametrize("where", ["device", "host"])
 def test_f32_accuracy(ctx, device, dtype, where):
     for opA, opB in (("C", "N"), ("N", "N"), ("T", "C")):
