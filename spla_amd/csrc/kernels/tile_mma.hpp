@@ -409,6 +409,76 @@ struct TileMma {
         stB2.load(op.B, op.ldb, col0, op.n, kBegin + BK, kEnd, tid);
       }
       __syncthreads();
+      if constexpr (kF64 && BK == 16 && !LEAN) {
+        // The split-barrier schedule of the large tiles with the loads two ahead:
+        // fragments in two register sets (sub-block 1's read during sub-block 0's
+        // MFMAs, the next k-block's sub-block 0 right after the barrier), the staged
+        // stores between the two halves of sub-block 0's MFMAs, sub-block 1's after
+        // the barrier. Stores and post-barrier reads are unconditional as there; a
+        // staging set never loaded (one k-block) stores zeros.
+        S fa0[2][FM], fa1[2][FM];
+        V2 fb[2][FN];
+        auto rd = [&](int set, int buf, int kb8) {
+#pragma unroll
+          for (int f = 0; f < FM; ++f) {
+            const int sr = CPLX ? (wm * WM) / 2 + 8 * f + cplx_row(l16) : wm * WM + 16 * f + l16;
+            const V2 v = lds_pair(&sm.lA[buf][sr][kb8 * 8 + 2 * kk]);
+            if (CPLX) {
+              const S x = swapPair ? v.y : v.x;
+              const S y = swapPair ? v.x : v.y;
+              fa0[set][f] = flip_sign(x, sgn0);
+              fa1[set][f] = flip_sign(y, sgn1);
+            } else {
+              fa0[set][f] = v.x;
+              fa1[set][f] = v.y;
+            }
+          }
+#pragma unroll
+          for (int g = 0; g < FN; ++g) {
+            const int j = wn * WN + 16 * g + l16;
+            fb[set][g] = lds_pair(&sm.lB[buf][j][kb8 * 8 + 2 * kk]);
+          }
+        };
+        auto mm = [&](int set, int half) {
+#pragma unroll
+          for (int f = 0; f < FM; ++f)
+#pragma unroll
+            for (int g = 0; g < FN; ++g)
+              acc[f][g] = half ? mfma16x16x4(fa1[set][f], fb[set][g].y, acc[f][g])
+                               : mfma16x16x4(fa0[set][f], fb[set][g].x, acc[f][g]);
+        };
+        if (nk == 1) {
+#pragma unroll
+          for (int u = 0; u < decltype(stA2)::kPerThread; ++u) stA2.regs[u] = V2{S(0), S(0)};
+#pragma unroll
+          for (int u = 0; u < decltype(stB2)::kPerThread; ++u) stB2.regs[u] = V2{S(0), S(0)};
+        }
+        if (nk > 0) rd(0, 0, 0);
+        auto kblock2 = [&](int it, auto& la, auto& lb, const auto& sa, const auto& sb) {
+          const int cur = it & 1;
+          if (it + 2 < nk) {
+            const int64_t q0 = kBegin + static_cast<int64_t>(it + 2) * BK;
+            la.load(op.A, op.lda, arow0, op.m, q0, kEnd, tid);
+            lb.load(op.B, op.ldb, col0, op.n, q0, kEnd, tid);
+          }
+          rd(1, cur, 1);
+          mm(0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          sa.store(sm.lA[cur ^ 1], tid);
+          sb.store(sm.lB[cur ^ 1], tid);
+          mm(0, 1);
+          __builtin_amdgcn_sched_barrier(0);
+          __syncthreads();
+          rd(0, cur ^ 1, 0);
+          mm(1, 0);
+          mm(1, 1);
+        };
+        for (int it = 0; it < nk; it += 2) {
+          kblock2(it, stA, stB, stA2, stB2);
+          if (it + 1 < nk) kblock2(it + 1, stA2, stB2, stA, stB);
+        }
+        return;
+      }
       // one k-block: (la, lb) receive k-block it + 2, (sa, sb) hold it + 1
       auto kblock = [&](int it, auto& la, auto& lb, const auto& sa, const auto& sb) {
         const int cur = it & 1;
